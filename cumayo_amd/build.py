@@ -1,0 +1,50 @@
+"""Builds the HIP engine: one shared library per MAYO parameter set.
+
+``python -m cumayo_amd.build`` compiles ``csrc/mayo_api.hip`` four times
+(``-DMAYO_VERSION=1,2,3,5``) for gfx950 into ``cumayo_amd/lib/``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc", "mayo_api.hip")
+LIBDIR = os.path.join(HERE, "lib")
+VERSIONS = (1, 2, 3, 5)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("CUMAYO_ARCH", "gfx950")
+
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed fp32 instructions disabled for all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def lib_path(version: int) -> str:
+    return os.path.join(LIBDIR, f"libcumayo_mayo{version}.so")
+
+
+def build_one(version: int, extra=()) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = lib_path(version)
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *FLAGS, f"-DMAYO_VERSION={version}", *extra, "-o", tmp, SRC]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for MAYO_{version}:\n{res.stderr[-4000:]}")
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(extra=()) -> list:
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        return list(ex.map(lambda v: build_one(v, extra), VERSIONS))
+
+
+if __name__ == "__main__":
+    for p in build_all(sys.argv[1:]):
+        print(p)

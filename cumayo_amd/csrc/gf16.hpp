@@ -1,0 +1,53 @@
+// GF(16) = GF(2)[x]/(x^4 + x + 1) on nibble-packed 32-bit words.
+//
+// A word holds 8 field elements, element e in bits [4e, 4e+4) -- the byte
+// order of the reference's m-vectors (low nibble first, code.h:7-11) read as
+// a little-endian word. A scalar-times-vector product is built from the four
+// multiples x, a*x, a^2*x, a^3*x of the vector (a = the class of x in the
+// field) and one mask per scalar bit, so each multiply-accumulate costs four
+// AND-XOR operations per word; the multiples of one vector are shared by
+// every scalar that multiplies it.
+#pragma once
+#include <cstdint>
+
+namespace gf16 {
+
+// Multiply each of the 8 nibbles by a (x^4 = x + 1).
+__device__ __forceinline__ uint32_t mul_a(uint32_t x) {
+    uint32_t h = (x & 0x88888888u) >> 3;
+    return ((x << 1) & 0xEEEEEEEEu) ^ h ^ (h << 1);
+}
+
+// All-ones if bit b of s is set, else zero.
+__device__ __forceinline__ uint32_t bitmask(uint32_t s, int b) {
+    return (uint32_t)(-(int32_t)((s >> b) & 1u));
+}
+
+struct Multiples {
+    uint32_t x0, x1, x2, x3;
+    __device__ __forceinline__ explicit Multiples(uint32_t x) {
+        x0 = x; x1 = mul_a(x0); x2 = mul_a(x1); x3 = mul_a(x2);
+    }
+    // s * x for a scalar nibble s
+    __device__ __forceinline__ uint32_t times(uint32_t s) const {
+        return (x0 & bitmask(s, 0)) ^ (x1 & bitmask(s, 1)) ^ (x2 & bitmask(s, 2)) ^
+               (x3 & bitmask(s, 3));
+    }
+};
+
+// Scalar product of two nibbles.
+__device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) {
+    Multiples m(a & 0xF);
+    return m.times(b & 0xF) & 0xF;
+}
+
+// Inverse via a^14 (a^15 = 1 for a != 0); 0 maps to 0.
+__device__ __forceinline__ uint32_t inv(uint32_t a) {
+    uint32_t a2 = mul(a, a), a4 = mul(a2, a2), a8 = mul(a4, a4);
+    return mul(mul(a8, a4), a2);
+}
+
+// Replicate a nibble into all 8 positions of a word.
+__device__ __forceinline__ uint32_t splat(uint32_t s) { return (s & 0xF) * 0x11111111u; }
+
+}  // namespace gf16

@@ -1,0 +1,302 @@
+// C ABI of the engine: the reference's pitched-pool entry points
+// (api.cu:17-218) and the batched key-table extensions.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/cumayo.h"
+#include "mayo_kernels.hpp"
+
+using namespace mayo;
+
+namespace {
+
+std::atomic<int> g_last_error{0};
+
+void note(hipError_t e) {
+    if (e != hipSuccess) g_last_error.store((int)e);
+}
+void note_launch() { note(hipGetLastError()); }
+
+constexpr size_t A128(size_t x) { return align128(x); }
+
+// AES tables, computed from the field definition (FIPS 197 section 5.1.1):
+// S-box = affine map of the GF(2^8) inverse; T0[x] = (2s, s, s, 3s).
+uint8_t gf256_mul(uint8_t a, uint8_t b) {
+    uint8_t r = 0;
+    while (b) {
+        if (b & 1) r ^= a;
+        a = (uint8_t)((a << 1) ^ ((a & 0x80) ? 0x1B : 0));
+        b >>= 1;
+    }
+    return r;
+}
+
+void build_aes_tables(uint32_t *T0, uint32_t *S) {
+    for (int a = 0; a < 256; ++a) {
+        uint8_t inv = 0;
+        for (int b = 1; a && b < 256; ++b)
+            if (gf256_mul((uint8_t)a, (uint8_t)b) == 1) { inv = (uint8_t)b; break; }
+        uint8_t s = inv, x = inv;
+        for (int i = 0; i < 4; ++i) { x = (uint8_t)((x << 1) | (x >> 7)); s ^= x; }
+        s ^= 0x63;
+        S[a] = s;
+        T0[a] = ((uint32_t)gf256_mul(s, 2) << 24) | ((uint32_t)s << 16) | ((uint32_t)s << 8) |
+                (uint32_t)gf256_mul(s, 3);
+    }
+}
+
+std::mutex g_init_mu;
+bool g_init_done[64] = {};
+
+void ensure_tables() {
+    int dev = 0;
+    note(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_init_done[dev]) return;
+    uint32_t T0[256], S[256];
+    build_aes_tables(T0, S);
+    note(hipMemcpyToSymbol(HIP_SYMBOL(c_aes_T0), T0, sizeof T0));
+    note(hipMemcpyToSymbol(HIP_SYMBOL(c_aes_S), S, sizeof S));
+    g_init_done[dev] = true;
+}
+
+inline unsigned blocks64(size_t n) { return (unsigned)((n + 63) / 64); }
+
+void copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows,
+            hipMemcpyKind kind, hipStream_t stream) {
+    if (!rows || !width) return;
+    note(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, stream));
+}
+
+// Pool offsets of the sign layouts (api.cu:97-104, 125-132).
+constexpr size_t SIGN_ESK_LEN = ESK_BYTES;
+constexpr size_t SIGNW_MSG_OFF = A128(ESK_BYTES);
+// Verify layouts (api.cu:147-151, 159-166).
+constexpr size_t EPK_LEN = (size_t)P_BYTES + PK_BYTES - 16;
+constexpr size_t EPK_PAD = A128(EPK_LEN);
+constexpr size_t SIG_PAD = A128(SIG_BYTES);
+
+void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) {
+    if (!n) return;
+    ensure_tables();
+    const size_t p1_off = SK + O_BYTES;
+    k_seed<<<blocks64(n), 64, 0, stream>>>(pool, pitch, n, 0, 0, 0, p1_off, SK);
+    note_launch();
+    k_aes_expand<<<(unsigned)n, WG, 0, stream>>>(pool, pitch, n, p1_off, p1_off);
+    note_launch();
+    k_expand_L<<<(unsigned)n, WG, 0, stream>>>(pool, pitch, n);
+    note_launch();
+}
+
+void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, size_t sig_off,
+                         size_t scr_off, size_t mlen, hipStream_t stream) {
+    if (!n) return;
+    PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
+    k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
+    note_launch();
+    k_sign_main<PitchedSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+    note_launch();
+}
+
+void launch_verify_pitched(uint8_t *pool, size_t pitch, size_t n, size_t mlen, hipStream_t stream) {
+    if (!n) return;
+    const size_t msg_off = EPK_PAD, sig_off = EPK_PAD + A128(mlen), res_off = sig_off + SIG_PAD;
+    PitchedVerify acc{pool, pitch, msg_off, sig_off, res_off, (uint32_t)mlen};
+    k_verify_main<PitchedVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+    note_launch();
+}
+
+}  // namespace
+
+extern "C" {
+
+void aes128_tables_init(void) { ensure_tables(); }
+
+void crypto_sign_keypair(uint8_t *pool, size_t pitch, size_t batch_size, hipStream_t stream,
+                         size_t rand_index) {
+    if (!batch_size) return;
+    ensure_tables();
+    const size_t pk_off = A128(SK), p_off = pk_off + A128(PK_BYTES);
+    // seed_pk lands in pk; O is parked in the P3 area until P3 replaces it
+    k_seed<<<blocks64(batch_size), 64, 0, stream>>>(pool, pitch, batch_size, 1, rand_index, 0,
+                                                    pk_off, pk_off + 16);
+    note_launch();
+    k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
+    note_launch();
+    k_keypair_P3<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
+    note_launch();
+}
+
+void crypto_sign_keypair_DtH(uint8_t *pk, uint8_t *sk, uint8_t *pool, size_t pitch,
+                             size_t batch_size, hipStream_t stream) {
+    copy2d(pk, PK_BYTES, pool + A128(SK), pitch, PK_BYTES, batch_size, hipMemcpyDeviceToHost, stream);
+    copy2d(sk, SK, pool, pitch, SK, batch_size, hipMemcpyDeviceToHost, stream);
+}
+
+void crypto_sign_expandsk_HtD(uint8_t *csk, uint8_t *pool, size_t pitch, size_t batch_size,
+                              hipStream_t stream) {
+    copy2d(pool, pitch, csk, SK, SK, batch_size, hipMemcpyHostToDevice, stream);
+}
+
+void crypto_sign_expandsk(uint8_t *pool, size_t pitch, size_t batch_size, hipStream_t stream) {
+    launch_expandsk(pool, pitch, batch_size, stream);
+}
+
+void crypto_sign_expandsk_DtH(uint8_t *esk, uint8_t *pool, size_t pitch, size_t batch_size,
+                              hipStream_t stream) {
+    copy2d(esk, ESK_BYTES, pool, pitch, ESK_BYTES, batch_size, hipMemcpyDeviceToHost, stream);
+}
+
+void crypto_sign_expandpk_HtD(uint8_t *cpk, uint8_t *pool, size_t pitch, size_t batch_size,
+                              hipStream_t stream) {
+    copy2d(pool + P_BYTES - 16, pitch, cpk, PK_BYTES, PK_BYTES, batch_size, hipMemcpyHostToDevice,
+           stream);
+}
+
+void crypto_sign_expandpk(uint8_t *pool, size_t pitch, size_t batch_size, hipStream_t stream) {
+    if (!batch_size) return;
+    ensure_tables();
+    k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, P_BYTES - 16, 0);
+    note_launch();
+}
+
+void crypto_sign_expandpk_DtH(uint8_t *epk, uint8_t *pool, size_t pitch, size_t batch_size,
+                              hipStream_t stream) {
+    copy2d(epk, EPK_LEN, pool, pitch, EPK_LEN, batch_size, hipMemcpyDeviceToHost, stream);
+}
+
+void crypto_sign_sign_HtD(uint8_t *cesk, uint8_t *cMess, size_t mlen, uint8_t *pool, size_t pitch,
+                          size_t batch_size, hipStream_t stream) {
+    copy2d(pool, pitch, cesk, SIGN_ESK_LEN, SIGN_ESK_LEN, batch_size, hipMemcpyHostToDevice, stream);
+    copy2d(pool + SIGN_ESK_LEN, pitch, cMess, mlen, mlen, batch_size, hipMemcpyHostToDevice, stream);
+}
+
+void crypto_sign_sign(uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
+                      hipStream_t stream) {
+    const size_t msg_off = SIGN_ESK_LEN, sig_off = msg_off + Mlen, scr_off = sig_off + SIG_BYTES;
+    launch_sign_pitched(pool, pitch, batch_size, msg_off, sig_off, scr_off, Mlen, stream);
+}
+
+void crypto_sign_sign_DtH(uint8_t *sig, uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
+                          hipStream_t stream) {
+    copy2d(sig, SIG_BYTES, pool + SIGN_ESK_LEN + Mlen, pitch, SIG_BYTES, batch_size,
+           hipMemcpyDeviceToHost, stream);
+}
+
+void crypto_sign_signWithesk_HtD(uint8_t *csk, uint8_t *cMess, size_t mlen, uint8_t *pool,
+                                 size_t pitch, size_t batch_size, hipStream_t stream) {
+    copy2d(pool, pitch, csk, SK, SK, batch_size, hipMemcpyHostToDevice, stream);
+    copy2d(pool + SIGNW_MSG_OFF, pitch, cMess, mlen, mlen, batch_size, hipMemcpyHostToDevice, stream);
+}
+
+void crypto_sign_signWithesk(uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
+                             hipStream_t stream) {
+    launch_expandsk(pool, pitch, batch_size, stream);
+    const size_t msg_off = SIGNW_MSG_OFF, sig_off = msg_off + A128(Mlen), scr_off = sig_off + SIG_PAD;
+    launch_sign_pitched(pool, pitch, batch_size, msg_off, sig_off, scr_off, Mlen, stream);
+}
+
+void crypto_sign_signWithesk_DtH(uint8_t *sig, uint8_t *pool, size_t Mlen, size_t pitch,
+                                 size_t batch_size, hipStream_t stream) {
+    copy2d(sig, SIG_BYTES, pool + SIGNW_MSG_OFF + A128(Mlen), pitch, SIG_BYTES, batch_size,
+           hipMemcpyDeviceToHost, stream);
+}
+
+void crypto_sign_verify_HtD(uint8_t *epk, uint8_t *Mess, uint8_t *sig, size_t mlen, uint8_t *pool,
+                            size_t pitch, size_t batch_size, hipStream_t stream) {
+    copy2d(pool, pitch, epk, EPK_PAD, EPK_PAD, batch_size, hipMemcpyHostToDevice, stream);
+    copy2d(pool + EPK_PAD, pitch, Mess, mlen, mlen, batch_size, hipMemcpyHostToDevice, stream);
+    copy2d(pool + EPK_PAD + A128(mlen), pitch, sig, SIG_PAD, SIG_PAD, batch_size,
+           hipMemcpyHostToDevice, stream);
+}
+
+void crypto_sign_verify(uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
+                        hipStream_t stream) {
+    launch_verify_pitched(pool, pitch, batch_size, Mlen, stream);
+}
+
+void crypto_sign_verify_DtH(uint8_t *verify, size_t Mlen, uint8_t *pool, size_t pitch,
+                            size_t batch_size, hipStream_t stream) {
+    copy2d(verify, 1, pool + EPK_PAD + A128(Mlen) + SIG_PAD, pitch, 1, batch_size,
+           hipMemcpyDeviceToHost, stream);
+}
+
+void crypto_sign_verifyWithepk_HtD(uint8_t *cpk, uint8_t *Mess, uint8_t *sig, size_t mlen,
+                                   uint8_t *pool, size_t pitch, size_t batch_size,
+                                   hipStream_t stream) {
+    copy2d(pool + P_BYTES - 16, pitch, cpk, PK_BYTES, PK_BYTES, batch_size, hipMemcpyHostToDevice,
+           stream);
+    copy2d(pool + EPK_PAD, pitch, Mess, mlen, mlen, batch_size, hipMemcpyHostToDevice, stream);
+    copy2d(pool + EPK_PAD + A128(mlen), pitch, sig, SIG_BYTES, SIG_BYTES, batch_size,
+           hipMemcpyHostToDevice, stream);
+}
+
+void crypto_sign_verifyWithepk(uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
+                               hipStream_t stream) {
+    if (!batch_size) return;
+    ensure_tables();
+    k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, P_BYTES - 16, 0);
+    note_launch();
+    launch_verify_pitched(pool, pitch, batch_size, Mlen, stream);
+}
+
+void crypto_sign_verifyWithepk_DtH(uint8_t *verify, size_t Mlen, uint8_t *pool, size_t pitch,
+                                   size_t batch_size, hipStream_t stream) {
+    crypto_sign_verify_DtH(verify, Mlen, pool, pitch, batch_size, stream);
+}
+
+void cumayo_get_sizes(cumayo_sizes *out) {
+    out->version = MAYO_VERSION;
+    out->n = N; out->m = M; out->o = O; out->k = K;
+    out->sk_bytes = SK; out->pk_bytes = PK_BYTES; out->sig_bytes = SIG_BYTES;
+    out->esk_bytes = ESK_BYTES; out->epk_bytes = EPK_BYTES; out->digest_bytes = DIGEST;
+    out->p1_bytes = P1_BYTES; out->p2_bytes = P2_BYTES; out->p3_bytes = P3_BYTES;
+    out->o_bytes = O_BYTES;
+}
+
+int cumayo_last_error(void) { return g_last_error.exchange(0); }
+
+size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_STRIDE; }
+
+int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
+                      const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                      uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream) {
+    if (!n) return 0;
+    if (!esk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !workspace)
+        return (int)hipErrorInvalidValue;
+    TableSign acc{esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, workspace};
+    k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        e = hipGetLastError();
+    }
+    note(e);
+    return (int)e;
+}
+
+size_t cumayo_verify_workspace_size(size_t n) { return n * (size_t)VERIFY_SCRATCH_STRIDE; }
+
+int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
+                        const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                        const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
+                        hipStream_t stream) {
+    if (!n) return 0;
+    if (!epk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !results || !workspace)
+        return (int)hipErrorInvalidValue;
+    TableVerify acc{epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs, results, workspace};
+    k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        e = hipGetLastError();
+    }
+    note(e);
+    return (int)e;
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+"""Python front end of the HIP MAYO engine (ctypes over the C ABI).
+
+``Engine(version)`` loads ``cumayo_amd/lib/libcumayo_mayo{version}.so`` and
+exposes
+
+* the reference's batched API (``include/api.cuh`` of cuMAYO) one-to-one:
+  ``crypto_sign_keypair``, ``crypto_sign_sign_HtD`` ... with device pools
+  given as torch uint8 tensors (rows ``pitch`` bytes apart) and host buffers
+  as numpy arrays, enqueued on a torch/HIP stream;
+* host-convenience wrappers that allocate pools, run a stage and copy the
+  result back (used by the tests);
+* the batched key-table entry points (``sign_table`` / ``verify_table``).
+
+There is no CPU fallback: constructing an Engine without the built library
+or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import build as _build
+
+VERSIONS = _build.VERSIONS
+
+
+def a128(x: int) -> int:
+    return (x + 127) // 128 * 128
+
+
+class _Sizes(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "version", "n", "m", "o", "k", "sk_bytes", "pk_bytes", "sig_bytes", "esk_bytes",
+        "epk_bytes", "digest_bytes", "p1_bytes", "p2_bytes", "p3_bytes", "o_bytes")]
+
+
+@dataclass(frozen=True)
+class Sizes:
+    version: int
+    n: int
+    m: int
+    o: int
+    k: int
+    sk_bytes: int
+    pk_bytes: int
+    sig_bytes: int
+    esk_bytes: int
+    epk_bytes: int
+    digest_bytes: int
+    p1_bytes: int
+    p2_bytes: int
+    p3_bytes: int
+    o_bytes: int
+
+    @property
+    def p_bytes(self) -> int:
+        return self.p1_bytes + self.p2_bytes
+
+    @property
+    def scratch_bytes(self) -> int:
+        # per-row scratch of the sign pools: k*m*ceil(o/2) (bench_stream.cu:82)
+        return self.k * self.m * ((self.o + 1) // 2)
+
+
+# (name, argtypes) of every symbol the header declares
+_VP = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+EXPORTS = {
+    "crypto_sign_keypair": [_VP, _SZ, _SZ, _VP, _SZ],
+    "crypto_sign_keypair_DtH": [_VP, _VP, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_expandsk_HtD": [_VP, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_expandsk": [_VP, _SZ, _SZ, _VP],
+    "crypto_sign_expandsk_DtH": [_VP, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_expandpk_HtD": [_VP, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_expandpk": [_VP, _SZ, _SZ, _VP],
+    "crypto_sign_expandpk_DtH": [_VP, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_sign_HtD": [_VP, _VP, _SZ, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_sign": [_VP, _SZ, _SZ, _SZ, _VP],
+    "crypto_sign_sign_DtH": [_VP, _VP, _SZ, _SZ, _SZ, _VP],
+    "crypto_sign_signWithesk_HtD": [_VP, _VP, _SZ, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_signWithesk": [_VP, _SZ, _SZ, _SZ, _VP],
+    "crypto_sign_signWithesk_DtH": [_VP, _VP, _SZ, _SZ, _SZ, _VP],
+    "crypto_sign_verify_HtD": [_VP, _VP, _VP, _SZ, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_verify": [_VP, _SZ, _SZ, _SZ, _VP],
+    "crypto_sign_verify_DtH": [_VP, _SZ, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_verifyWithepk_HtD": [_VP, _VP, _VP, _SZ, _VP, _SZ, _SZ, _VP],
+    "crypto_sign_verifyWithepk": [_VP, _SZ, _SZ, _SZ, _VP],
+    "crypto_sign_verifyWithepk_DtH": [_VP, _SZ, _VP, _SZ, _SZ, _VP],
+    "aes128_tables_init": [],
+    "cumayo_get_sizes": [ctypes.POINTER(_Sizes)],
+    "cumayo_last_error": [],
+    "cumayo_sign_workspace_size": [_SZ],
+    "cumayo_sign_table": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
+    "cumayo_verify_workspace_size": [_SZ],
+    "cumayo_verify_table": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
+}
+_RESTYPES = {
+    "cumayo_last_error": ctypes.c_int,
+    "cumayo_sign_workspace_size": _SZ,
+    "cumayo_verify_workspace_size": _SZ,
+    "cumayo_sign_table": ctypes.c_int,
+    "cumayo_verify_table": ctypes.c_int,
+}
+
+
+def load_library(version: int) -> ctypes.CDLL:
+    """Loads the per-set library and binds every exported symbol."""
+    path = _build.lib_path(version)
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP engine library missing: {path} (run python -m cumayo_amd.build)")
+    lib = ctypes.CDLL(path)
+    for name, args in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, None)
+    return lib
+
+
+def _ptr(x) -> int:
+    """Address of a torch tensor, numpy array or raw int."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    return x.data_ptr()
+
+
+class Engine:
+    def __init__(self, version: int, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("cumayo_amd needs a ROCm GPU; none is visible")
+        self.torch = torch
+        self.version = version
+        self.device = torch.device("cuda", device)
+        self.lib = load_library(version)
+        s = _Sizes()
+        self.lib.cumayo_get_sizes(ctypes.byref(s))
+        self.sizes = Sizes(**{f[0]: getattr(s, f[0]) for f in _Sizes._fields_})
+        if self.sizes.version != version:
+            raise RuntimeError("library/parameter-set mismatch")
+        with torch.cuda.device(self.device):
+            self.lib.aes128_tables_init()
+        self.check()
+
+    # -- plumbing ---------------------------------------------------------
+    def stream_handle(self, stream=None) -> int:
+        if stream is None:
+            stream = self.torch.cuda.current_stream(self.device)
+        return stream.cuda_stream
+
+    def check(self):
+        err = self.lib.cumayo_last_error()
+        if err:
+            raise RuntimeError(f"HIP error {err} in cumayo_amd (MAYO_{self.version})")
+
+    def pool(self, rows: int, row_bytes: int):
+        pitch = (row_bytes + 255) // 256 * 256
+        t = self.torch.zeros((max(rows, 1), pitch), dtype=self.torch.uint8, device=self.device)
+        return t, pitch
+
+    # -- host convenience wrappers (allocate, run, copy back, synchronise) --
+    def keypair(self, batch: int, rand_index: int = 0):
+        z = self.sizes
+        row = a128(z.sk_bytes) + a128(z.pk_bytes) + a128(z.p_bytes)
+        pool, pitch = self.pool(batch, row)
+        pk = np.zeros((batch, z.pk_bytes), np.uint8)
+        sk = np.zeros((batch, z.sk_bytes), np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_keypair(pool.data_ptr(), pitch, batch, st, rand_index)
+        self.lib.crypto_sign_keypair_DtH(pk.ctypes.data, sk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return pk, sk
+
+    def expandsk(self, sk: np.ndarray) -> np.ndarray:
+        z = self.sizes
+        sk = np.ascontiguousarray(sk, np.uint8).reshape(-1, z.sk_bytes)
+        batch = sk.shape[0]
+        pool, pitch = self.pool(batch, z.esk_bytes)
+        esk = np.zeros((batch, z.esk_bytes), np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_expandsk_HtD(sk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandsk(pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandsk_DtH(esk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return esk
+
+    def expandpk(self, pk: np.ndarray) -> np.ndarray:
+        z = self.sizes
+        pk = np.ascontiguousarray(pk, np.uint8).reshape(-1, z.pk_bytes)
+        batch = pk.shape[0]
+        pool, pitch = self.pool(batch, z.epk_bytes)
+        epk = np.zeros((batch, z.epk_bytes), np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_expandpk_HtD(pk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandpk(pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandpk_DtH(epk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return epk
+
+    def sign(self, esk: np.ndarray, msgs: np.ndarray) -> np.ndarray:
+        """crypto_sign_sign over rows esk || msg || sig || scratch."""
+        z = self.sizes
+        esk = np.ascontiguousarray(esk, np.uint8).reshape(-1, z.esk_bytes)
+        msgs = np.ascontiguousarray(msgs, np.uint8).reshape(esk.shape[0], -1)
+        batch, mlen = msgs.shape
+        pool, pitch = self.pool(batch, z.esk_bytes + mlen + z.sig_bytes + z.scratch_bytes)
+        sig = np.zeros((batch, z.sig_bytes), np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_sign_HtD(esk.ctypes.data, msgs.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_sign(pool.data_ptr(), mlen, pitch, batch, st)
+        self.lib.crypto_sign_sign_DtH(sig.ctypes.data, pool.data_ptr(), mlen, pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return sig
+
+    def sign_with_sk(self, sk: np.ndarray, msgs: np.ndarray) -> np.ndarray:
+        """crypto_sign_signWithesk: compact sk in, expanded on the GPU, signed."""
+        z = self.sizes
+        sk = np.ascontiguousarray(sk, np.uint8).reshape(-1, z.sk_bytes)
+        msgs = np.ascontiguousarray(msgs, np.uint8).reshape(sk.shape[0], -1)
+        batch, mlen = msgs.shape
+        row = a128(z.esk_bytes) + a128(mlen) + a128(z.sig_bytes) + a128(z.scratch_bytes)
+        pool, pitch = self.pool(batch, row)
+        sig = np.zeros((batch, z.sig_bytes), np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_signWithesk_HtD(sk.ctypes.data, msgs.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_signWithesk(pool.data_ptr(), mlen, pitch, batch, st)
+        self.lib.crypto_sign_signWithesk_DtH(sig.ctypes.data, pool.data_ptr(), mlen, pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return sig
+
+    def verify(self, epk: np.ndarray, msgs: np.ndarray, sigs: np.ndarray) -> np.ndarray:
+        """crypto_sign_verify: host epk and sig rows A128-padded (api.cu:148-155)."""
+        z = self.sizes
+        epk_len = z.p_bytes + z.pk_bytes - 16
+        epk = np.ascontiguousarray(epk, np.uint8).reshape(-1, epk_len)
+        batch = epk.shape[0]
+        msgs = np.ascontiguousarray(msgs, np.uint8).reshape(batch, -1)
+        mlen = msgs.shape[1]
+        epk_pad = np.zeros((batch, a128(epk_len)), np.uint8)
+        epk_pad[:, :epk_len] = epk
+        sig_pad = np.zeros((batch, a128(z.sig_bytes)), np.uint8)
+        sig_pad[:, :z.sig_bytes] = np.asarray(sigs, np.uint8).reshape(batch, -1)
+        row = a128(epk_len) + a128(mlen) + a128(z.sig_bytes) + 1
+        pool, pitch = self.pool(batch, row)
+        res = np.full(batch, 0xEE, np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_verify_HtD(epk_pad.ctypes.data, msgs.ctypes.data, sig_pad.ctypes.data, mlen,
+                                        pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_verify(pool.data_ptr(), mlen, pitch, batch, st)
+        self.lib.crypto_sign_verify_DtH(res.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return res
+
+    def verify_with_pk(self, pk: np.ndarray, msgs: np.ndarray, sigs: np.ndarray) -> np.ndarray:
+        """crypto_sign_verifyWithepk: compact pk in, expanded on the GPU, verified."""
+        z = self.sizes
+        epk_len = z.p_bytes + z.pk_bytes - 16
+        pk = np.ascontiguousarray(pk, np.uint8).reshape(-1, z.pk_bytes)
+        batch = pk.shape[0]
+        msgs = np.ascontiguousarray(msgs, np.uint8).reshape(batch, -1)
+        mlen = msgs.shape[1]
+        sigs = np.ascontiguousarray(sigs, np.uint8).reshape(batch, z.sig_bytes)
+        row = a128(epk_len) + a128(mlen) + a128(z.sig_bytes) + 1
+        pool, pitch = self.pool(batch, row)
+        res = np.full(batch, 0xEE, np.uint8)
+        st = self.stream_handle()
+        self.lib.crypto_sign_verifyWithepk_HtD(pk.ctypes.data, msgs.ctypes.data, sigs.ctypes.data, mlen,
+                                               pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_verifyWithepk(pool.data_ptr(), mlen, pitch, batch, st)
+        self.lib.crypto_sign_verifyWithepk_DtH(res.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.torch.cuda.synchronize(self.device)
+        self.check()
+        return res
+
+    # -- batched key-table form (device tensors in, enqueue only) ----------
+    def sign_workspace(self, n: int):
+        return self.torch.empty(max(int(self.lib.cumayo_sign_workspace_size(n)), 1),
+                                dtype=self.torch.uint8, device=self.device)
+
+    def verify_workspace(self, n: int):
+        return self.torch.empty(max(int(self.lib.cumayo_verify_workspace_size(n)), 1),
+                                dtype=self.torch.uint8, device=self.device)
+
+    def sign_table(self, esk_table, key_idx, msgs, msg_off, msg_len, sigs, workspace, stream=None):
+        z = self.sizes
+        n = int(key_idx.numel())
+        err = self.lib.cumayo_sign_table(esk_table.data_ptr(), esk_table.stride(0) if esk_table.dim() > 1 else z.esk_bytes,
+                                         key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
+                                         msg_len.data_ptr(), sigs.data_ptr(), n, workspace.data_ptr(),
+                                         self.stream_handle(stream))
+        if err:
+            raise RuntimeError(f"cumayo_sign_table failed: HIP error {err}")
+
+    def verify_table(self, epk_table, key_idx, msgs, msg_off, msg_len, sigs, results, workspace,
+                     stream=None):
+        z = self.sizes
+        n = int(key_idx.numel())
+        err = self.lib.cumayo_verify_table(epk_table.data_ptr(), epk_table.stride(0) if epk_table.dim() > 1 else z.epk_bytes,
+                                           key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
+                                           msg_len.data_ptr(), sigs.data_ptr(), results.data_ptr(), n,
+                                           workspace.data_ptr(), self.stream_handle(stream))
+        if err:
+            raise RuntimeError(f"cumayo_verify_table failed: HIP error {err}")

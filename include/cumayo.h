@@ -1,0 +1,134 @@
+/*
+ * cumayo.h -- C ABI of the MI355X MAYO engine (one shared library per
+ * parameter set: libcumayo_mayo1.so, libcumayo_mayo2.so, libcumayo_mayo3.so,
+ * libcumayo_mayo5.so, mirroring the reference's one build per MAYO_VERSION,
+ * params.h:6-8 and tests/CMakeLists.txt:28-45).
+ *
+ * Part 1 is a drop-in for include/api.cuh of cuMAYO: the same 20 entry
+ * points with the same names, argument order, pitched device pool layouts and
+ * stream semantics (cudaStream_t becomes hipStream_t), plus
+ * aes128_tables_init (aes128_ctr.cuh:16). Like the reference these calls
+ * return void, never synchronise and only enqueue work on `stream`; unlike
+ * the reference, the last launch or copy error is kept and can be read with
+ * cumayo_last_error().
+ *
+ * Part 2 extends the boundary with the batched key-table form used for
+ * shared resident keys and ragged messages; it has no counterpart in the
+ * reference.
+ */
+#ifndef CUMAYO_H
+#define CUMAYO_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+#define CUMAYO_DEFAULT(x) = x
+extern "C" {
+#else
+#define CUMAYO_DEFAULT(x)
+#endif
+
+/* ---------------- Part 1: drop-in for api.cuh ---------------- */
+
+/* api.cuh:9 -- generates batch_size key pairs into pitched pool rows:
+ * sk at 0, pk at A128(sk), P1||P2 scratch after A128(pk) (api.cu:17-26).
+ * The seed of row i is LE64(rand_index + 2 i + 1) (keypair.cu:32-40). */
+void crypto_sign_keypair(uint8_t *d_keypair_mem_pool, size_t keypair_mem_pool_pitch,
+                         size_t batch_size CUMAYO_DEFAULT(1), hipStream_t stream CUMAYO_DEFAULT(nullptr),
+                         size_t rand_index CUMAYO_DEFAULT(0));
+/* api.cuh:11 (api.cu:29-38) */
+void crypto_sign_keypair_DtH(uint8_t *pk, uint8_t *sk, uint8_t *d_keypair_mem_pool, size_t pitch,
+                             size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* api.cuh:13-17 (api.cu:40-62): compact sk in, esk = sk||O||P1||L out */
+void crypto_sign_expandsk_HtD(uint8_t *csk, uint8_t *d_expandsk_mem_pool, size_t pitch,
+                              size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_expandsk(uint8_t *d_expandsk_mem_pool, size_t pitch, size_t batch_size,
+                          hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_expandsk_DtH(uint8_t *esk, uint8_t *d_expandsk_mem_pool, size_t pitch,
+                              size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* api.cuh:19-23 (api.cu:64-85): compact pk placed at P_BYTES-16, epk = P1||P2||P3 */
+void crypto_sign_expandpk_HtD(uint8_t *cpk, uint8_t *d_expandpk_mem_pool, size_t pitch,
+                              size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_expandpk(uint8_t *d_expandpk_mem_pool, size_t pitch, size_t batch_size,
+                          hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_expandpk_DtH(uint8_t *epk, uint8_t *d_expandpk_mem_pool, size_t pitch,
+                              size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* api.cuh:25-29 (api.cu:88-114): rows esk || msg || sig || scratch
+ * (scratch of k*m*ceil(o/2) bytes, as the reference's d_M) */
+void crypto_sign_sign_HtD(uint8_t *cesk, uint8_t *cMess, size_t mlen, uint8_t *d_sign_mem_pool,
+                          size_t pitch, size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_sign(uint8_t *d_sign_mem_pool, size_t Mlen, size_t pitch, size_t batch_size,
+                      hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_sign_DtH(uint8_t *sig, uint8_t *d_sig_mem_pool, size_t Mlen, size_t pitch,
+                          size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* api.cuh:31-35 (api.cu:116-144): compact sk in, expanded then signed;
+ * rows A128(esk) || A128(msg) || A128(sig) || scratch */
+void crypto_sign_signWithesk_HtD(uint8_t *cesk, uint8_t *cMess, size_t mlen,
+                                 uint8_t *d_sign_mem_pool, size_t pitch, size_t batch_size,
+                                 hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_signWithesk(uint8_t *d_sign_mem_pool, size_t Mlen, size_t pitch,
+                             size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_signWithesk_DtH(uint8_t *sig, uint8_t *d_sig_mem_pool, size_t Mlen, size_t pitch,
+                                 size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* api.cuh:37-41 (api.cu:146-178): rows A128(epk) || A128(msg) || A128(sig) || result;
+ * host epk and sig rows are A128-padded, as in the reference */
+void crypto_sign_verify_HtD(uint8_t *epk, uint8_t *Mess, uint8_t *sig, size_t mlen,
+                            uint8_t *d_verify_mem_pool, size_t pitch, size_t batch_size,
+                            hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_verify(uint8_t *d_verify_mem_pool, size_t Mlen, size_t pitch, size_t batch_size,
+                        hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_verify_DtH(uint8_t *verify, size_t Mlen, uint8_t *d_verify_mem_pool, size_t pitch,
+                            size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* api.cuh:43-47 (api.cu:180-218): compact pk in, expanded then verified;
+ * result byte 0 = valid, 1 = invalid (verify.cu:209-211) */
+void crypto_sign_verifyWithepk_HtD(uint8_t *epk, uint8_t *Mess, uint8_t *sig, size_t mlen,
+                                   uint8_t *d_verify_mem_pool, size_t pitch, size_t batch_size,
+                                   hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_verifyWithepk(uint8_t *d_verify_mem_pool, size_t Mlen, size_t pitch,
+                               size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
+void crypto_sign_verifyWithepk_DtH(uint8_t *verify, size_t Mlen, uint8_t *d_verify_mem_pool,
+                                   size_t pitch, size_t batch_size,
+                                   hipStream_t stream CUMAYO_DEFAULT(nullptr));
+/* aes128_ctr.cuh:16 (aes128_ctr.cu:69-72): uploads the AES tables to the
+ * current device. The entry points above also do it on first use. */
+void aes128_tables_init(void);
+
+/* ---------------- Part 2: engine extensions ---------------- */
+
+typedef struct {
+    int version, n, m, o, k;
+    int sk_bytes, pk_bytes, sig_bytes, esk_bytes, epk_bytes, digest_bytes;
+    int p1_bytes, p2_bytes, p3_bytes, o_bytes;
+} cumayo_sizes;
+
+/* Parameter set compiled into this library. */
+void cumayo_get_sizes(cumayo_sizes *out);
+/* Last HIP error recorded by any entry point of this library (0 = none),
+ * and its name; reading clears it. */
+int cumayo_last_error(void);
+
+/* Batched signing over a table of expanded secret keys (reference esk
+ * format, rows esk_stride bytes apart): item i signs msg_len[i] bytes at
+ * msgs + msg_off[i] with key key_idx[i] into sigs + i*sig_bytes. All
+ * pointers are device pointers; workspace holds
+ * cumayo_sign_workspace_size(n) bytes. Returns 0 or a hipError_t. */
+size_t cumayo_sign_workspace_size(size_t n);
+int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
+                      const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                      uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream);
+
+/* Batched verification over a table of expanded public keys (P1||P2||P3);
+ * results[i] = 0 valid, 1 invalid. */
+size_t cumayo_verify_workspace_size(size_t n);
+int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
+                        const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                        const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
+                        hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
