@@ -1,0 +1,198 @@
+"""Benchmark: MAYO_1 batched sign and verify on one MI355X.
+
+Workload (the headline configuration): 65,536 signatures under 64 shared
+keys (key j mod 64) whose expanded secret and public keys stay resident in
+HBM, messages of uniform random length in [32, 1024] bytes. One step is one
+pass of the hot path over the whole batch: ``sign`` signs all 65,536
+messages, ``verify`` verifies all 65,536 signatures. Inputs are generated
+from SHAKE streams and are in HBM before the timed region.
+
+Prints one JSON line. ``value`` is signatures per second; verification
+throughput is reported beside it. ``roofline`` covers the signing
+linear-algebra kernel (the dominant kernel), timed with HIP events on the
+stream it runs on. ``cpu_baseline`` times the CPU oracle (the restatement of
+the reference algorithm) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Work per operation (GF(16) multiply-accumulates), from the reference loop
+# bounds: sign counts one attempt (sign.cu:256-437 plus the elimination),
+# verify counts verify.cu:72-182.
+SIGN_MACS = {1: 3_677_947, 2: 982_688, 3: 10_147_914, 5: 24_133_255}
+VERIFY_MACS = {1: 4_197_180, 2: 998_656, 3: 11_294_316, 5: 26_389_848}
+# Nominal INT roofline: 256 CU x 64 lanes x 2.4 GHz 32-bit ops, 32 GF(16)
+# MACs per 16 bitsliced AND-XOR ops -> 2 MACs per op.
+PEAK_GF16_TMACS = 256 * 64 * 2.4e9 * 2 / 1e12
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(n: int, nkeys: int, lo: int, hi: int, tag: str = "cfg2"):
+    """Messages j: SHAKE256(tag/msg || LE64(j)); lengths from SHAKE256(tag/len || LE64(j))."""
+    lens = np.empty(n, np.int32)
+    for j in range(n):
+        r = int.from_bytes(hashlib.shake_256(f"mayo-bench/{tag}/len".encode() + j.to_bytes(8, "little")).digest(8), "little")
+        lens[j] = lo + r % (hi - lo + 1)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    flat = bytearray(int(lens.sum()))
+    for j in range(n):
+        o = int(offs[j])
+        flat[o:o + int(lens[j])] = hashlib.shake_256(f"mayo-bench/{tag}/msg".encode() + j.to_bytes(8, "little")).digest(int(lens[j]))
+    key_idx = (np.arange(n) % nkeys).astype(np.int32)
+    return np.frombuffer(bytes(flat), np.uint8), offs, lens, key_idx
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--keys", type=int, default=64)
+    ap.add_argument("--version", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=4096,
+                    help="signatures the CPU baseline signs (bounded sample)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine is measured on one GPU; multi-GPU runs are not "
+                          "part of this build (items shard by key with no collective)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from cumayo_amd import Engine
+
+    v = args.version
+    eng = Engine(v)
+    z = eng.sizes
+    dev = eng.device
+    n = args.batch
+    print(f"[bench] MAYO_{v} batch={n} keys={args.keys}: generating inputs", file=sys.stderr, flush=True)
+    pk, sk = eng.keypair(args.keys, rand_index=0)
+    esk = eng.expandsk(sk)
+    epk = eng.expandpk(pk)
+    flat, offs, lens, kidx = make_workload(n, args.keys, 32, 1024)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d_esk, d_epk = t(esk), t(epk)
+    d_msgs, d_off, d_len, d_kidx = t(flat), t(offs), t(lens), t(kidx)
+    d_sigs = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+    d_res = torch.full((n,), 0xEE, dtype=torch.uint8, device=dev)
+    ws_s = eng.sign_workspace(n)
+    ws_v = eng.verify_workspace(n)
+    stream = torch.cuda.current_stream(dev)
+
+    def sign_step(ev=None):
+        if ev is None:
+            eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s)
+        else:
+            eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s, stages=1)
+            ev[0].record(stream)
+            eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s, stages=2)
+            ev[1].record(stream)
+
+    def verify_step():
+        eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_v)
+
+    for _ in range(args.warmup):
+        sign_step()
+        verify_step()
+    torch.cuda.synchronize(dev)
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sign_step(events[k])
+    torch.cuda.synchronize(dev)
+    sign_s = (time.perf_counter() - t0) / args.steps
+    main_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        verify_step()
+    torch.cuda.synchronize(dev)
+    verify_s = (time.perf_counter() - t0) / args.steps
+    print(f"[bench] timed: sign {sign_s*1e3:.2f} ms/step verify {verify_s*1e3:.2f} ms/step",
+          file=sys.stderr, flush=True)
+
+    res = d_res.cpu().numpy()
+    all_verify = bool((res == 0).all())
+    sigs = d_sigs.cpu().numpy()
+
+    sign_per_s = n / sign_s
+    verify_per_s = n / verify_s
+    achieved_tmacs = n * SIGN_MACS[v] / (main_ms * 1e-3) / 1e12
+    # algorithmic HBM bytes of the main kernel per signature: the message-side
+    # reads are in the hashing pass; the main pass reads V||r, t (scratch) and
+    # the esk blocks it touches once per signature (L, P1, O) and writes the
+    # signature. Keys are shared, so this is an upper bound on compulsory bytes.
+    algo_bytes = n * (z.sig_bytes + z.k * z.n // 2 + z.m // 2)
+    out = {
+        "metric": "MAYO_1 sign/s & verify/s at batch 64K (bit-exact); serve p50/p99 latency",
+        "value": round(sign_per_s, 1),
+        "unit": "sign/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sign_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf16 (u4 nibbles in u32 words)",
+        "data": "synthetic (SHAKE-seeded messages, keys from the reference seed rule)",
+        "config": {"workload": f"MAYO_{v} sign+verify, {n} signatures, {args.keys} resident keys "
+                               f"(key j mod {args.keys}), message lengths uniform [32,1024] B",
+                   "global_batch": n, "keys": args.keys, "param_set": f"MAYO_{v}"},
+        "verify": {"value": round(verify_per_s, 1), "unit": "verify/s",
+                   "ms_per_step": round(verify_s * 1e3, 3), "all_valid": all_verify},
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_sign_main (signing linear algebra)",
+            "achieved": round(achieved_tmacs, 3),
+            "peak": round(PEAK_GF16_TMACS, 1),
+            "unit": "T GF16-MAC/s",
+            "frac": round(achieved_tmacs / PEAK_GF16_TMACS, 4),
+            "kernel_ms": round(main_ms, 3),
+            "macs_per_sign": SIGN_MACS[v],
+            "hbm_algorithmic_GBs": round(algo_bytes / (main_ms * 1e-3) / 1e9, 2),
+            "hbm_peak_GBs": HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        S = min(args.cpu_sample, n)
+        print(f"[bench] cpu baseline: oracle sign of {S} items on {threads} threads",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        osigs = O.sign_batch(v, esk, kidx[:S], flat, offs[:S], lens[:S], threads=threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(S / cpu_s, 1), "unit": "sign/s", "cores": threads, "kind": "port",
+            "sample": f"first {S} signatures of the same workload, oracle (C restatement of the "
+                      f"reference algorithm, OpenMP over items)",
+            "bit_exact_vs_gpu": bool(np.array_equal(osigs, sigs[:S])),
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -262,16 +262,20 @@ int cumayo_last_error(void) { return g_last_error.exchange(0); }
 
 size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_STRIDE; }
 
-int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
-                      const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
-                      uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream) {
+int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
+                             const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                             uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
+                             int stages) {
     if (!n) return 0;
     if (!esk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !workspace)
         return (int)hipErrorInvalidValue;
     TableSign acc{esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, workspace};
-    k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
+    hipError_t e = hipSuccess;
+    if (stages & 1) {
+        k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && (stages & 2)) {
         k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         e = hipGetLastError();
     }
@@ -279,24 +283,42 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
     return (int)e;
 }
 
+int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
+                      const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                      uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream) {
+    return cumayo_sign_table_stages(esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, n,
+                                    workspace, stream, 3);
+}
+
 size_t cumayo_verify_workspace_size(size_t n) { return n * (size_t)VERIFY_SCRATCH_STRIDE; }
 
-int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
-                        const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
-                        const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
-                        hipStream_t stream) {
+int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
+                               const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                               const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
+                               hipStream_t stream, int stages) {
     if (!n) return 0;
     if (!epk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !results || !workspace)
         return (int)hipErrorInvalidValue;
     TableVerify acc{epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs, results, workspace};
-    k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
+    hipError_t e = hipSuccess;
+    if (stages & 1) {
+        k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && (stages & 2)) {
         k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         e = hipGetLastError();
     }
     note(e);
     return (int)e;
+}
+
+int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
+                        const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                        const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
+                        hipStream_t stream) {
+    return cumayo_verify_table_stages(epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs,
+                                      results, n, workspace, stream, 3);
 }
 
 }  // extern "C"

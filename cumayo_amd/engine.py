@@ -96,6 +96,8 @@ EXPORTS = {
     "cumayo_sign_table": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
     "cumayo_verify_workspace_size": [_SZ],
     "cumayo_verify_table": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
+    "cumayo_sign_table_stages": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
+    "cumayo_verify_table_stages": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
 }
 _RESTYPES = {
     "cumayo_last_error": ctypes.c_int,
@@ -103,6 +105,8 @@ _RESTYPES = {
     "cumayo_verify_workspace_size": _SZ,
     "cumayo_sign_table": ctypes.c_int,
     "cumayo_verify_table": ctypes.c_int,
+    "cumayo_sign_table_stages": ctypes.c_int,
+    "cumayo_verify_table_stages": ctypes.c_int,
 }
 
 
@@ -293,23 +297,25 @@ class Engine:
         return self.torch.empty(max(int(self.lib.cumayo_verify_workspace_size(n)), 1),
                                 dtype=self.torch.uint8, device=self.device)
 
-    def sign_table(self, esk_table, key_idx, msgs, msg_off, msg_len, sigs, workspace, stream=None):
+    def sign_table(self, esk_table, key_idx, msgs, msg_off, msg_len, sigs, workspace, stream=None,
+                   stages=3):
         z = self.sizes
         n = int(key_idx.numel())
-        err = self.lib.cumayo_sign_table(esk_table.data_ptr(), esk_table.stride(0) if esk_table.dim() > 1 else z.esk_bytes,
-                                         key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
-                                         msg_len.data_ptr(), sigs.data_ptr(), n, workspace.data_ptr(),
-                                         self.stream_handle(stream))
+        err = self.lib.cumayo_sign_table_stages(
+            esk_table.data_ptr(), esk_table.stride(0) if esk_table.dim() > 1 else z.esk_bytes,
+            key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(), msg_len.data_ptr(),
+            sigs.data_ptr(), n, workspace.data_ptr(), self.stream_handle(stream), stages)
         if err:
             raise RuntimeError(f"cumayo_sign_table failed: HIP error {err}")
 
     def verify_table(self, epk_table, key_idx, msgs, msg_off, msg_len, sigs, results, workspace,
-                     stream=None):
+                     stream=None, stages=3):
         z = self.sizes
         n = int(key_idx.numel())
-        err = self.lib.cumayo_verify_table(epk_table.data_ptr(), epk_table.stride(0) if epk_table.dim() > 1 else z.epk_bytes,
-                                           key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
-                                           msg_len.data_ptr(), sigs.data_ptr(), results.data_ptr(), n,
-                                           workspace.data_ptr(), self.stream_handle(stream))
+        err = self.lib.cumayo_verify_table_stages(
+            epk_table.data_ptr(), epk_table.stride(0) if epk_table.dim() > 1 else z.epk_bytes,
+            key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(), msg_len.data_ptr(),
+            sigs.data_ptr(), results.data_ptr(), n, workspace.data_ptr(),
+            self.stream_handle(stream), stages)
         if err:
             raise RuntimeError(f"cumayo_verify_table failed: HIP error {err}")

@@ -119,6 +119,15 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
                       const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
                       uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream);
 
+/* The same in two stages, selected by the bit mask `stages`: 1 = the
+ * hashing pass (digest, salt, target, vinegar vector of attempt 0, one lane
+ * per item), 2 = the linear-algebra pass (one workgroup per item). Stage 2
+ * reads what stage 1 left in the workspace; 3 runs both. */
+int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
+                             const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                             uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
+                             int stages);
+
 /* Batched verification over a table of expanded public keys (P1||P2||P3);
  * results[i] = 0 valid, 1 invalid. */
 size_t cumayo_verify_workspace_size(size_t n);
@@ -126,6 +135,11 @@ int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32
                         const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
                         const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
                         hipStream_t stream);
+
+int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
+                               const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                               const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
+                               hipStream_t stream, int stages);
 
 #ifdef __cplusplus
 }
