@@ -65,7 +65,20 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="signatures the CPU baseline signs (bounded sample)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel()}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU; multi-GPU runs are not "
                           "part of this build (items shard by key with no collective)",
@@ -81,12 +94,11 @@ def main():
     dev = eng.device
     n = args.batch
     print(f"[bench] MAYO_{v} batch={n} keys={args.keys}: generating inputs", file=sys.stderr, flush=True)
-    pk, sk = eng.keypair(args.keys, rand_index=0)
-    esk = eng.expandsk(sk)
-    epk = eng.expandpk(pk)
+    # keys: generated, then expanded straight into dense device tables
+    # (pitch = row length), so the resident keys never leave HBM
+    d_esk, d_epk = eng.keypair_tables(args.keys, rand_index=0)
     flat, offs, lens, kidx = make_workload(n, args.keys, 32, 1024)
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    d_esk, d_epk = t(esk), t(epk)
+    t = lambda a: torch.from_numpy(np.array(a)).to(dev)
     d_msgs, d_off, d_len, d_kidx = t(flat), t(offs), t(lens), t(kidx)
     d_sigs = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
     d_res = torch.full((n,), 0xEE, dtype=torch.uint8, device=dev)
@@ -133,7 +145,9 @@ def main():
 
     res = d_res.cpu().numpy()
     all_verify = bool((res == 0).all())
+    bad = np.where(res != 0)[0]
     sigs = d_sigs.cpu().numpy()
+    attempts = eng.sign_attempts(ws_s, n)
 
     sign_per_s = n / sign_s
     verify_per_s = n / verify_s
@@ -160,7 +174,11 @@ def main():
                                f"(key j mod {args.keys}), message lengths uniform [32,1024] B",
                    "global_batch": n, "keys": args.keys, "param_set": f"MAYO_{v}"},
         "verify": {"value": round(verify_per_s, 1), "unit": "verify/s",
-                   "ms_per_step": round(verify_s * 1e3, 3), "all_valid": all_verify},
+                   "ms_per_step": round(verify_s * 1e3, 3), "all_valid": all_verify,
+                   "invalid": int(bad.size), "invalid_first": bad[:8].tolist()},
+        "sign_retries": {"items_with_retry": int((attempts > 1).sum()),
+                         "exhausted": int((attempts == 0).sum()),
+                         "max_attempts": int(attempts.max())},
         "roofline": {
             "bound": "valu",
             "kernel": "k_sign_main (signing linear algebra)",
@@ -183,6 +201,7 @@ def main():
         print(f"[bench] cpu baseline: oracle sign of {S} items on {threads} threads",
               file=sys.stderr, flush=True)
         t0 = time.perf_counter()
+        esk = d_esk.cpu().numpy()
         osigs = O.sign_batch(v, esk, kidx[:S], flat, offs[:S], lens[:S], threads=threads)
         cpu_s = time.perf_counter() - t0
         out["cpu_baseline"] = {

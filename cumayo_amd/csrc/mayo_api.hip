@@ -256,6 +256,8 @@ void cumayo_get_sizes(cumayo_sizes *out) {
     out->esk_bytes = ESK_BYTES; out->epk_bytes = EPK_BYTES; out->digest_bytes = DIGEST;
     out->p1_bytes = P1_BYTES; out->p2_bytes = P2_BYTES; out->p3_bytes = P3_BYTES;
     out->o_bytes = O_BYTES;
+    out->sign_ws_stride = SIGN_SCRATCH_STRIDE;
+    out->sign_ws_attempts = SCR_ATTEMPTS;
 }
 
 int cumayo_last_error(void) { return g_last_error.exchange(0); }

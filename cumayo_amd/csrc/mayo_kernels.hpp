@@ -32,7 +32,8 @@ constexpr int SCR_DIGEST = 0;
 constexpr int SCR_SALT = DIGEST;
 constexpr int SCR_T = DIGEST + SK;
 constexpr int SCR_VR = DIGEST + SK + MB;
-constexpr int SIGN_SCRATCH = SCR_VR + VR_BYTES;
+constexpr int SCR_ATTEMPTS = SCR_VR + VR_BYTES;  // attempts used (0 = all failed)
+constexpr int SIGN_SCRATCH = SCR_ATTEMPTS + 1;
 constexpr int SIGN_SCRATCH_STRIDE = (SIGN_SCRATCH + 63) / 64 * 64;
 // Scratch per verifying item: t.
 constexpr int VERIFY_SCRATCH_STRIDE = (MB + 15) / 16 * 16;
@@ -608,7 +609,10 @@ __global__ void __launch_bounds__(WG) k_sign_main(Acc acc, size_t n) {
         }
         __syncthreads();
         const int ok = sample_solution(S);
-        if (ok || ctr == MAX_ATTEMPTS - 1) break;
+        if (ok || ctr == MAX_ATTEMPTS - 1) {
+            if (tid == 0) it.scratch[SCR_ATTEMPTS] = ok ? (uint8_t)(ctr + 1) : 0;
+            break;
+        }
         // rare path: next attempt's V || r (sign.cu:251-252)
         __syncthreads();
         if (tid == 0) {

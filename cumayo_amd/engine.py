@@ -34,7 +34,8 @@ def a128(x: int) -> int:
 class _Sizes(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "version", "n", "m", "o", "k", "sk_bytes", "pk_bytes", "sig_bytes", "esk_bytes",
-        "epk_bytes", "digest_bytes", "p1_bytes", "p2_bytes", "p3_bytes", "o_bytes")]
+        "epk_bytes", "digest_bytes", "p1_bytes", "p2_bytes", "p3_bytes", "o_bytes",
+        "sign_ws_stride", "sign_ws_attempts")]
 
 
 @dataclass(frozen=True)
@@ -54,6 +55,8 @@ class Sizes:
     p2_bytes: int
     p3_bytes: int
     o_bytes: int
+    sign_ws_stride: int
+    sign_ws_attempts: int
 
     @property
     def p_bytes(self) -> int:
@@ -135,6 +138,9 @@ def _ptr(x) -> int:
 
 
 class Engine:
+    """Host wrappers stage every host buffer through pinned memory: the
+    asynchronous 2-D copies of the C ABI are then plain DMA transfers."""
+
     def __init__(self, version: int, device: int = 0):
         import torch
         if not torch.cuda.is_available():
@@ -163,6 +169,12 @@ class Engine:
         if err:
             raise RuntimeError(f"HIP error {err} in cumayo_amd (MAYO_{self.version})")
 
+    def pinned(self, shape, src=None):
+        t = self.torch.empty(shape, dtype=self.torch.uint8, pin_memory=True)
+        if src is not None:
+            t.numpy()[...] = np.asarray(src, np.uint8).reshape(shape)
+        return t
+
     def pool(self, rows: int, row_bytes: int):
         pitch = (row_bytes + 255) // 256 * 256
         t = self.torch.zeros((max(rows, 1), pitch), dtype=self.torch.uint8, device=self.device)
@@ -173,42 +185,44 @@ class Engine:
         z = self.sizes
         row = a128(z.sk_bytes) + a128(z.pk_bytes) + a128(z.p_bytes)
         pool, pitch = self.pool(batch, row)
-        pk = np.zeros((batch, z.pk_bytes), np.uint8)
-        sk = np.zeros((batch, z.sk_bytes), np.uint8)
+        pk = self.pinned((batch, z.pk_bytes))
+        sk = self.pinned((batch, z.sk_bytes))
         st = self.stream_handle()
         self.lib.crypto_sign_keypair(pool.data_ptr(), pitch, batch, st, rand_index)
-        self.lib.crypto_sign_keypair_DtH(pk.ctypes.data, sk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_keypair_DtH(pk.data_ptr(), sk.data_ptr(), pool.data_ptr(), pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return pk, sk
+        return pk.numpy().copy(), sk.numpy().copy()
 
     def expandsk(self, sk: np.ndarray) -> np.ndarray:
         z = self.sizes
         sk = np.ascontiguousarray(sk, np.uint8).reshape(-1, z.sk_bytes)
         batch = sk.shape[0]
         pool, pitch = self.pool(batch, z.esk_bytes)
-        esk = np.zeros((batch, z.esk_bytes), np.uint8)
+        h_sk = self.pinned(sk.shape, sk)
+        esk = self.pinned((batch, z.esk_bytes))
         st = self.stream_handle()
-        self.lib.crypto_sign_expandsk_HtD(sk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandsk_HtD(h_sk.data_ptr(), pool.data_ptr(), pitch, batch, st)
         self.lib.crypto_sign_expandsk(pool.data_ptr(), pitch, batch, st)
-        self.lib.crypto_sign_expandsk_DtH(esk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandsk_DtH(esk.data_ptr(), pool.data_ptr(), pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return esk
+        return esk.numpy().copy()
 
     def expandpk(self, pk: np.ndarray) -> np.ndarray:
         z = self.sizes
         pk = np.ascontiguousarray(pk, np.uint8).reshape(-1, z.pk_bytes)
         batch = pk.shape[0]
         pool, pitch = self.pool(batch, z.epk_bytes)
-        epk = np.zeros((batch, z.epk_bytes), np.uint8)
+        h_pk = self.pinned(pk.shape, pk)
+        epk = self.pinned((batch, z.epk_bytes))
         st = self.stream_handle()
-        self.lib.crypto_sign_expandpk_HtD(pk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandpk_HtD(h_pk.data_ptr(), pool.data_ptr(), pitch, batch, st)
         self.lib.crypto_sign_expandpk(pool.data_ptr(), pitch, batch, st)
-        self.lib.crypto_sign_expandpk_DtH(epk.ctypes.data, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_expandpk_DtH(epk.data_ptr(), pool.data_ptr(), pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return epk
+        return epk.numpy().copy()
 
     def sign(self, esk: np.ndarray, msgs: np.ndarray) -> np.ndarray:
         """crypto_sign_sign over rows esk || msg || sig || scratch."""
@@ -217,14 +231,15 @@ class Engine:
         msgs = np.ascontiguousarray(msgs, np.uint8).reshape(esk.shape[0], -1)
         batch, mlen = msgs.shape
         pool, pitch = self.pool(batch, z.esk_bytes + mlen + z.sig_bytes + z.scratch_bytes)
-        sig = np.zeros((batch, z.sig_bytes), np.uint8)
+        h_esk, h_msg = self.pinned(esk.shape, esk), self.pinned(msgs.shape, msgs)
+        sig = self.pinned((batch, z.sig_bytes))
         st = self.stream_handle()
-        self.lib.crypto_sign_sign_HtD(esk.ctypes.data, msgs.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_sign_HtD(h_esk.data_ptr(), h_msg.data_ptr(), mlen, pool.data_ptr(), pitch, batch, st)
         self.lib.crypto_sign_sign(pool.data_ptr(), mlen, pitch, batch, st)
-        self.lib.crypto_sign_sign_DtH(sig.ctypes.data, pool.data_ptr(), mlen, pitch, batch, st)
+        self.lib.crypto_sign_sign_DtH(sig.data_ptr(), pool.data_ptr(), mlen, pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return sig
+        return sig.numpy().copy()
 
     def sign_with_sk(self, sk: np.ndarray, msgs: np.ndarray) -> np.ndarray:
         """crypto_sign_signWithesk: compact sk in, expanded on the GPU, signed."""
@@ -234,14 +249,15 @@ class Engine:
         batch, mlen = msgs.shape
         row = a128(z.esk_bytes) + a128(mlen) + a128(z.sig_bytes) + a128(z.scratch_bytes)
         pool, pitch = self.pool(batch, row)
-        sig = np.zeros((batch, z.sig_bytes), np.uint8)
+        h_sk, h_msg = self.pinned(sk.shape, sk), self.pinned(msgs.shape, msgs)
+        sig = self.pinned((batch, z.sig_bytes))
         st = self.stream_handle()
-        self.lib.crypto_sign_signWithesk_HtD(sk.ctypes.data, msgs.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_signWithesk_HtD(h_sk.data_ptr(), h_msg.data_ptr(), mlen, pool.data_ptr(), pitch, batch, st)
         self.lib.crypto_sign_signWithesk(pool.data_ptr(), mlen, pitch, batch, st)
-        self.lib.crypto_sign_signWithesk_DtH(sig.ctypes.data, pool.data_ptr(), mlen, pitch, batch, st)
+        self.lib.crypto_sign_signWithesk_DtH(sig.data_ptr(), pool.data_ptr(), mlen, pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return sig
+        return sig.numpy().copy()
 
     def verify(self, epk: np.ndarray, msgs: np.ndarray, sigs: np.ndarray) -> np.ndarray:
         """crypto_sign_verify: host epk and sig rows A128-padded (api.cu:148-155)."""
@@ -257,15 +273,17 @@ class Engine:
         sig_pad[:, :z.sig_bytes] = np.asarray(sigs, np.uint8).reshape(batch, -1)
         row = a128(epk_len) + a128(mlen) + a128(z.sig_bytes) + 1
         pool, pitch = self.pool(batch, row)
-        res = np.full(batch, 0xEE, np.uint8)
+        res = self.pinned((batch,), np.full(batch, 0xEE, np.uint8))
+        h_epk, h_msg, h_sig = (self.pinned(epk_pad.shape, epk_pad), self.pinned(msgs.shape, msgs),
+                               self.pinned(sig_pad.shape, sig_pad))
         st = self.stream_handle()
-        self.lib.crypto_sign_verify_HtD(epk_pad.ctypes.data, msgs.ctypes.data, sig_pad.ctypes.data, mlen,
+        self.lib.crypto_sign_verify_HtD(h_epk.data_ptr(), h_msg.data_ptr(), h_sig.data_ptr(), mlen,
                                         pool.data_ptr(), pitch, batch, st)
         self.lib.crypto_sign_verify(pool.data_ptr(), mlen, pitch, batch, st)
-        self.lib.crypto_sign_verify_DtH(res.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_verify_DtH(res.data_ptr(), mlen, pool.data_ptr(), pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return res
+        return res.numpy().copy()
 
     def verify_with_pk(self, pk: np.ndarray, msgs: np.ndarray, sigs: np.ndarray) -> np.ndarray:
         """crypto_sign_verifyWithepk: compact pk in, expanded on the GPU, verified."""
@@ -278,17 +296,69 @@ class Engine:
         sigs = np.ascontiguousarray(sigs, np.uint8).reshape(batch, z.sig_bytes)
         row = a128(epk_len) + a128(mlen) + a128(z.sig_bytes) + 1
         pool, pitch = self.pool(batch, row)
-        res = np.full(batch, 0xEE, np.uint8)
+        res = self.pinned((batch,), np.full(batch, 0xEE, np.uint8))
+        h_pk, h_msg, h_sig = (self.pinned(pk.shape, pk), self.pinned(msgs.shape, msgs),
+                              self.pinned(sigs.shape, sigs))
         st = self.stream_handle()
-        self.lib.crypto_sign_verifyWithepk_HtD(pk.ctypes.data, msgs.ctypes.data, sigs.ctypes.data, mlen,
+        self.lib.crypto_sign_verifyWithepk_HtD(h_pk.data_ptr(), h_msg.data_ptr(), h_sig.data_ptr(), mlen,
                                                pool.data_ptr(), pitch, batch, st)
         self.lib.crypto_sign_verifyWithepk(pool.data_ptr(), mlen, pitch, batch, st)
-        self.lib.crypto_sign_verifyWithepk_DtH(res.ctypes.data, mlen, pool.data_ptr(), pitch, batch, st)
+        self.lib.crypto_sign_verifyWithepk_DtH(res.data_ptr(), mlen, pool.data_ptr(), pitch, batch, st)
         self.torch.cuda.synchronize(self.device)
         self.check()
-        return res
+        return res.numpy().copy()
+
+    def keypair_tables(self, nkeys: int, rand_index: int = 0):
+        """Generates nkeys key pairs and expands them into dense device
+        tables (esk rows, epk rows) without leaving the device."""
+        z = self.sizes
+        torch = self.torch
+        row = a128(z.sk_bytes) + a128(z.pk_bytes) + a128(z.p_bytes)
+        pool, pitch = self.pool(nkeys, row)
+        st = self.stream_handle()
+        self.lib.crypto_sign_keypair(pool.data_ptr(), pitch, nkeys, st, rand_index)
+        esk = torch.zeros((nkeys, z.esk_bytes), dtype=torch.uint8, device=self.device)
+        epk = torch.zeros((nkeys, z.epk_bytes), dtype=torch.uint8, device=self.device)
+        pk_off = a128(z.sk_bytes)
+        esk[:, :z.sk_bytes].copy_(pool[:nkeys, :z.sk_bytes])
+        epk[:, z.p_bytes - 16:].copy_(pool[:nkeys, pk_off:pk_off + z.pk_bytes])
+        self.lib.crypto_sign_expandsk(esk.data_ptr(), z.esk_bytes, nkeys, st)
+        self.lib.crypto_sign_expandpk(epk.data_ptr(), z.epk_bytes, nkeys, st)
+        torch.cuda.synchronize(self.device)
+        self.check()
+        return esk, epk
+
+    def expand_tables(self, sk: np.ndarray, pk: np.ndarray):
+        """Expanded key tables on the device: esk rows and epk rows packed
+        densely (pool pitch = row length), built by crypto_sign_expandsk /
+        crypto_sign_expandpk without a host round trip."""
+        z = self.sizes
+        torch = self.torch
+        sk = np.ascontiguousarray(sk, np.uint8).reshape(-1, z.sk_bytes)
+        pk = np.ascontiguousarray(pk, np.uint8).reshape(-1, z.pk_bytes)
+        nk = sk.shape[0]
+        st = self.stream_handle()
+        h_sk, h_pk = self.pinned(sk.shape, sk), self.pinned(pk.shape, pk)
+        esk = torch.zeros((nk, z.esk_bytes), dtype=torch.uint8, device=self.device)
+        self.lib.crypto_sign_expandsk_HtD(h_sk.data_ptr(), esk.data_ptr(), z.esk_bytes, nk, st)
+        self.lib.crypto_sign_expandsk(esk.data_ptr(), z.esk_bytes, nk, st)
+        epk = torch.zeros((nk, z.epk_bytes), dtype=torch.uint8, device=self.device)
+        self.lib.crypto_sign_expandpk_HtD(h_pk.data_ptr(), epk.data_ptr(), z.epk_bytes, nk, st)
+        self.lib.crypto_sign_expandpk(epk.data_ptr(), z.epk_bytes, nk, st)
+        torch.cuda.synchronize(self.device)
+        self.check()
+        return esk, epk
 
     # -- batched key-table form (device tensors in, enqueue only) ----------
+    @property
+    def sign_attempts_offset(self) -> int:
+        return self.sizes.sign_ws_attempts
+
+    def sign_attempts(self, workspace, n: int) -> np.ndarray:
+        """Attempts each item of the last sign_table call used (0 = exhausted)."""
+        z = self.sizes
+        return workspace[: n * z.sign_ws_stride].view(n, z.sign_ws_stride)[:, z.sign_ws_attempts].cpu().numpy()
+
     def sign_workspace(self, n: int):
         return self.torch.empty(max(int(self.lib.cumayo_sign_workspace_size(n)), 1),
                                 dtype=self.torch.uint8, device=self.device)

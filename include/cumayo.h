@@ -101,6 +101,10 @@ typedef struct {
     int version, n, m, o, k;
     int sk_bytes, pk_bytes, sig_bytes, esk_bytes, epk_bytes, digest_bytes;
     int p1_bytes, p2_bytes, p3_bytes, o_bytes;
+    /* batch-form workspace: bytes per signing item, and the offset within
+     * an item's slot of the byte that reports the attempts used (1..255,
+     * 0 when every attempt failed) */
+    int sign_ws_stride, sign_ws_attempts;
 } cumayo_sizes;
 
 /* Parameter set compiled into this library. */
