@@ -181,7 +181,7 @@ def main():
                          "max_attempts": int(attempts.max())},
         "roofline": {
             "bound": "valu",
-            "kernel": "k_sign_main (signing linear algebra)",
+            "kernel": "k_sign_wave (signing linear algebra, one wavefront per signature)",
             "achieved": round(achieved_tmacs, 3),
             "peak": round(PEAK_GF16_TMACS, 1),
             "unit": "T GF16-MAC/s",

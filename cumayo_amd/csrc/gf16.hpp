@@ -35,6 +35,21 @@ struct Multiples {
     }
 };
 
+// acc ^ (x & m) in one v_bitop3_b32 (truth table 0x78 over (acc, x, m)).
+__device__ __forceinline__ uint32_t axm(uint32_t acc, uint32_t x, uint32_t m) {
+    return __builtin_amdgcn_bitop3_b32(acc, x, m, 0x78);
+}
+
+// acc ^= s * x given the multiples of x and the four bit masks of s.
+__device__ __forceinline__ uint32_t mac(uint32_t acc, uint32_t x0, uint32_t x1, uint32_t x2,
+                                        uint32_t x3, uint32_t m0, uint32_t m1, uint32_t m2,
+                                        uint32_t m3) {
+    acc = axm(acc, x0, m0);
+    acc = axm(acc, x1, m1);
+    acc = axm(acc, x2, m2);
+    return axm(acc, x3, m3);
+}
+
 // Scalar product of two nibbles.
 __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) {
     Multiples m(a & 0xF);

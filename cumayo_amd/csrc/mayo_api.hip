@@ -8,6 +8,7 @@
 
 #include "../../include/cumayo.h"
 #include "mayo_kernels.hpp"
+#include "mayo_wave.hpp"
 
 using namespace mayo;
 
@@ -98,7 +99,7 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
-    k_sign_main<PitchedSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+    wv::k_sign_wave<PitchedSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }
 
@@ -106,7 +107,7 @@ void launch_verify_pitched(uint8_t *pool, size_t pitch, size_t n, size_t mlen, h
     if (!n) return;
     const size_t msg_off = EPK_PAD, sig_off = EPK_PAD + A128(mlen), res_off = sig_off + SIG_PAD;
     PitchedVerify acc{pool, pitch, msg_off, sig_off, res_off, (uint32_t)mlen};
-    k_verify_main<PitchedVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+    wv::k_verify_wave<PitchedVerify><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }
 
@@ -278,7 +279,10 @@ int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const 
         e = hipGetLastError();
     }
     if (e == hipSuccess && (stages & 2)) {
-        k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        if (stages & 4)
+            k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        else
+            wv::k_sign_wave<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
         e = hipGetLastError();
     }
     note(e);
@@ -308,7 +312,10 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
         e = hipGetLastError();
     }
     if (e == hipSuccess && (stages & 2)) {
-        k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        if (stages & 4)
+            k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        else
+            wv::k_verify_wave<TableVerify><<<(unsigned)n, 64, 0, stream>>>(acc, n);
         e = hipGetLastError();
     }
     note(e);

@@ -1,0 +1,576 @@
+// Wave-per-item signing and verification kernels.
+//
+// One wavefront (64 lanes, one workgroup) owns one signature end to end, so
+// the per-signature scalars (the vinegar / signature nibbles) are uniform
+// across the wave and their bit masks live in SGPRs. Phases:
+//
+//  M      M_i[.][x] = sum_r V_i[r] L[r][x]: lanes = (column x, word w) items,
+//         r uniform.
+//  U      U_ij = V_i^T P1 V_j + V_j^T P1 V_i: lanes = (word w, row slot);
+//         the column index c is uniform, every lane accumulates T_j[r] for a
+//         chunk of its rows, then folds T into the pair accumulators.
+//  y, A   sums of z^l-shifted m-vectors, accumulated unreduced (degree
+//         < m + pairs) and reduced once by z^m = F0 + F1 z + F2 z^2 + F3 z^3.
+//  solve  Gauss-Jordan on the columns of [A | y - A r], each column held
+//         bitsliced (4 bit-planes over the rows) in one lane's registers; the
+//         pivot column is broadcast through SGPRs. The pivot is the lowest
+//         unused row with a nonzero entry; x = r + z with z supported on the
+//         pivot columns, identical to the reference's echelon form plus
+//         back-substitution because that solution is unique.
+#pragma once
+#include "keccak_wave.hpp"
+#include "mayo_kernels.hpp"
+
+namespace mayo {
+namespace wv {
+
+constexpr int SL = 64 / MW;                    // quad-form row slots per wave
+constexpr int NG = (M + 31) / 32;              // 32-row groups of a bitsliced column
+constexpr int NCOL = KO + 1;                   // columns of [A | y']
+constexpr int NCL = (NCOL + 63) / 64;          // columns held per lane
+constexpr int OVW = (PAIRS + 7) / 8 + 1;       // overflow words of an unreduced sum
+constexpr int UW = MW + OVW;                   // words of an unreduced sum
+constexpr int RT = 2;                          // quad-form rows per chunk per lane
+// column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
+// go to the last set so that only one set stays active for most pivots
+constexpr int SHIFT = NCOL - 64 * (NCL - 1);
+
+__host__ __device__ constexpr int col_of(int set, int lane) {
+    // set NCL-1 holds columns [0, SHIFT) (its other lanes hold nothing: an
+    // index past NCOL); sets 0..NCL-2 hold [SHIFT + 64 s, SHIFT + 64 s + 64)
+    return set == NCL - 1 ? (lane < SHIFT ? lane : NCOL + lane) : SHIFT + 64 * set + lane;
+}
+// order in which the sets' columns come: set NCL-1 first, then 0, 1, ...
+__host__ __device__ constexpr int set_rank(int set) { return set == NCL - 1 ? 0 : set + 1; }
+// the y' column (index KO = NCOL - 1) is the last lane of set NCL-2
+constexpr int Y_SET = NCL - 2;
+constexpr int Y_LANE = 63;
+static_assert(NCL >= 2 && col_of(Y_SET, Y_LANE) == KO, "y' column placement");
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+    return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | (uint64_t)uni((uint32_t)x);
+}
+
+// Multiply a bitsliced nibble vector (planes p0..p3) by a.
+__device__ __forceinline__ void bs_mul_a(uint32_t &p0, uint32_t &p1, uint32_t &p2, uint32_t &p3) {
+    uint32_t t = p3;
+    p3 = p2; p2 = p1; p1 = p0 ^ t; p0 = t;
+}
+
+// Bit d of the 8 nibbles of w, packed into 8 consecutive bits.
+__device__ __forceinline__ uint32_t nib_plane8(uint32_t w, int d) {
+    uint32_t x = (w >> d) & 0x11111111u;
+    x = (x | (x >> 3)) & 0x03030303u;
+    x = (x | (x >> 6)) & 0x000F000Fu;
+    return (x | (x >> 12)) & 0xFFu;
+}
+
+struct Shared {
+    uint8_t vr[VR_BYTES];
+    uint64_t scol[N];            // scol[c] nibble j = S_j[c]
+    uint32_t Mv[K * O * MW];     // M_i column x as an m-vector
+    uint32_t U[PAIRS * MW];
+    uint32_t yu[UW];             // unreduced y
+    uint32_t y[MW];              // y = t + sum E^l u_l
+    uint32_t t[MW];
+    uint32_t Ar[4 * NG];         // A r, bitsliced
+    uint32_t yplanes[4 * NG];    // final y' column, bitsliced
+    int16_t prow[KO];            // pivot row of each column, -1 if free
+    uint8_t x[KO];
+    uint8_t mul[256];            // GF(16) product table, a*16 + b
+    uint8_t hbuf[DIGEST + 2 * SK + 1];  // hash input staging
+};
+
+// ---- quadratic form: Ulds[pair][w] ^= sum_r S_a[r] T_j[r] -------------------
+// P is the n x n form [[P1, P2], [0, P3]] (FULL) or P1 alone (v x v).
+template <bool FULL>
+__device__ __forceinline__ void quad_form(const uint8_t *P1, const uint8_t *P2, const uint8_t *P3,
+                                          const uint64_t *scol, uint32_t *Ulds) {
+    constexpr int NR = FULL ? N : V;
+    constexpr int RPS = (NR + SL - 1) / SL;  // rows per slot
+    const int lane = threadIdx.x;
+    const int w = lane % MW, slot = lane / MW;
+    const bool active = slot < SL;
+    const int nv = mvec::word_valid(w);
+    uint32_t U[PAIRS];
+#pragma unroll
+    for (int p = 0; p < PAIRS; ++p) U[p] = 0;
+    for (int t0 = 0; t0 < RPS; t0 += RT) {
+        uint32_t T[RT][K];
+#pragma unroll
+        for (int tt = 0; tt < RT; ++tt)
+#pragma unroll
+            for (int j = 0; j < K; ++j) T[tt][j] = 0;
+        const int cstart = t0 * SL;
+#pragma unroll 1
+        for (int c = cstart; c < NR; ++c) {
+            const uint64_t sc = uni64(scol[c]);
+            // multiples of this lane's entries (r, c) of the chunk; a row not
+            // yet reached (r > c) or past the end contributes zero
+            uint32_t x0[RT], x1[RT], x2[RT], x3[RT];
+#pragma unroll
+            for (int tt = 0; tt < RT; ++tt) {
+                const int r = slot + SL * (t0 + tt);
+                uint32_t xv = 0;
+                if (active && r <= c && r < NR) {
+                    const uint8_t *e;
+                    if (!FULL || (r < V && c < V)) e = P1 + (size_t)tri_entry(r, c, V) * MB;
+                    else if (r < V) e = P2 + (size_t)(r * O + (c - V)) * MB;
+                    else e = P3 + (size_t)tri_entry(r - V, c - V, O) * MB;
+                    xv = mvec::load_word(e + 4 * w, nv);
+                }
+                x0[tt] = xv; x1[tt] = gf16::mul_a(xv); x2[tt] = gf16::mul_a(x1[tt]); x3[tt] = gf16::mul_a(x2[tt]);
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const uint32_t sj = (uint32_t)(sc >> (4 * j));
+                const uint32_t m0 = gf16::bitmask(sj, 0), m1 = gf16::bitmask(sj, 1);
+                const uint32_t m2 = gf16::bitmask(sj, 2), m3 = gf16::bitmask(sj, 3);
+#pragma unroll
+                for (int tt = 0; tt < RT; ++tt)
+                    T[tt][j] = gf16::mac(T[tt][j], x0[tt], x1[tt], x2[tt], x3[tt], m0, m1, m2, m3);
+            }
+        }
+#pragma unroll
+        for (int tt = 0; tt < RT; ++tt) {
+            const int r = slot + SL * (t0 + tt);
+            if (active && r < NR) {
+                const uint64_t sr = scol[r];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    gf16::Multiples mt(T[tt][j]);
+#pragma unroll
+                    for (int a = 0; a < K; ++a) {
+                        const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
+                        const uint32_t sa = (uint32_t)(sr >> (4 * a));
+                        U[p] = gf16::mac(U[p], mt.x0, mt.x1, mt.x2, mt.x3, gf16::bitmask(sa, 0),
+                                         gf16::bitmask(sa, 1), gf16::bitmask(sa, 2),
+                                         gf16::bitmask(sa, 3));
+                    }
+                }
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int p = 0; p < PAIRS; ++p) atomicXor(&Ulds[p * MW + w], U[p]);
+    }
+}
+
+// ---- unreduced shifted sums ---------------------------------------------------
+// acc (UW words) ^= z^ell * src (an m-vector of MW words in LDS), no reduction.
+__device__ __forceinline__ void add_shift(uint32_t (&acc)[UW], const uint32_t *src, int ell) {
+    const int ws = ell >> 3, ns = 4 * (ell & 7);
+#pragma unroll
+    for (int q = 0; q < UW; ++q) {
+        const int hi = q - ws, lo = q - ws - 1;
+        uint32_t h = (hi >= 0 && hi < MW) ? src[hi] : 0u;
+        uint32_t l = (lo >= 0 && lo < MW) ? src[lo] : 0u;
+        acc[q] ^= ns ? __builtin_amdgcn_alignbit(h, l, 32 - ns) : h;
+    }
+}
+
+// Reduce an unreduced sum modulo f(z) into MW words: the nibbles at degree
+// m + t (t < pairs) fold back as z^t (F0 + F1 z + F2 z^2 + F3 z^3).
+__device__ __forceinline__ void reduce_unreduced(const uint32_t (&acc)[UW], uint32_t (&out)[MW]) {
+    constexpr int QS = M / 8, QN = 4 * (M % 8);
+    uint32_t ov[OVW + 1];
+#pragma unroll
+    for (int k = 0; k <= OVW; ++k) {
+        const int a = QS + k, b = QS + k + 1;
+        uint32_t lo = a < UW ? acc[a] : 0u;
+        uint32_t hi = b < UW ? acc[b] : 0u;
+        ov[k] = QN ? __builtin_amdgcn_alignbit(hi, lo, QN) : lo;
+    }
+#pragma unroll
+    for (int q = 0; q < MW; ++q) out[q] = acc[q];
+    out[MW - 1] &= mvec::LAST_MASK;
+#pragma unroll
+    for (int k = 0; k <= OVW && k < MW; ++k) {
+        gf16::Multiples mo(ov[k]);
+        const uint32_t t0 = mo.times(F0), t1 = mo.times(F1), t2 = mo.times(F2), t3 = mo.times(F3);
+        out[k] ^= t0 ^ (t1 << 4) ^ (t2 << 8) ^ (t3 << 12);
+        if (k + 1 < MW) out[k + 1] ^= (t1 >> 28) ^ (t2 >> 24) ^ (t3 >> 20);
+    }
+}
+
+// Nibble-packed m-vector (MW words) -> 4 bit-planes of NG words over rows.
+__device__ __forceinline__ void to_planes(const uint32_t (&v)[MW], uint32_t (&pl)[4][NG]) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            uint32_t o = 0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int wi = 4 * g + s;
+                if (wi < MW) o |= nib_plane8(v[wi], d) << (8 * s);
+            }
+            pl[d][g] = o;
+        }
+}
+
+// Horner order of the pair (i, j): l(i, j) in the A schedule (sign.cu:336-392).
+__device__ __forceinline__ int ell_of(int i, int j) {
+    // rows i' < i contribute K - i' pairs each; within row i, j runs K-1 down
+    return i * K - i * (i - 1) / 2 + (K - 1 - j);
+}
+
+// y = t + sum_l E^l u_l computed by the wave; result in S.y (MW words).
+__device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
+    const int lane = threadIdx.x;
+    if (lane < UW) {
+        uint32_t acc = 0;
+        for (int i = 0; i < K; ++i)
+            for (int j = i; j < K; ++j) {
+                const int ell = ell_of(i, j), ws = ell >> 3, ns = 4 * (ell & 7);
+                const uint32_t *src = &S.U[pair_index(i, j) * MW];
+                const int hi = lane - ws, lo = lane - ws - 1;
+                uint32_t h = (hi >= 0 && hi < MW) ? src[hi] : 0u;
+                uint32_t l = (lo >= 0 && lo < MW) ? src[lo] : 0u;
+                acc ^= ns ? __builtin_amdgcn_alignbit(h, l, 32 - ns) : h;
+            }
+        S.yu[lane] = acc;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        uint32_t acc[UW], out[MW];
+#pragma unroll
+        for (int q = 0; q < UW; ++q) acc[q] = S.yu[q];
+        reduce_unreduced(acc, out);
+#pragma unroll
+        for (int q = 0; q < MW; ++q) S.y[q] = out[q] ^ t[q];
+    }
+    __syncthreads();
+}
+
+// GF(16) product of a per-lane nibble and a uniform nibble via the LDS table.
+__device__ __forceinline__ uint32_t tmul(const Shared &S, uint32_t a, uint32_t b) {
+    return S.mul[(a << 4) | b];
+}
+
+// Build the A columns this lane owns (bitsliced) and the y' column.
+// Returns in cols[NCL][4][NG]; columns past KO (other than y') are zero.
+__device__ __forceinline__ void build_columns(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < NCL; ++s) {
+        const int c = col_of(s, lane);
+        uint32_t v[MW];
+#pragma unroll
+        for (int q = 0; q < MW; ++q) v[q] = 0;
+        if (c < KO) {
+            const int b = c / O, x = c % O;
+            uint32_t acc[UW];
+#pragma unroll
+            for (int q = 0; q < UW; ++q) acc[q] = 0;
+            // pairs (b, j), j >= b: block b gets z^l M_j; pairs (i, b), i < b: z^l M_i
+            for (int j = b; j < K; ++j) add_shift(acc, &S.Mv[(j * O + x) * MW], ell_of(b, j));
+            for (int i = 0; i < b; ++i) add_shift(acc, &S.Mv[(i * O + x) * MW], ell_of(i, b));
+            reduce_unreduced(acc, v);
+        } else if (c == KO) {
+#pragma unroll
+            for (int q = 0; q < MW; ++q) v[q] = S.y[q];
+        }
+        to_planes(v, cols[s]);
+    }
+}
+
+// Gauss-Jordan solve; x (S.x) = r + z. Returns 1 on full rank.
+__device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
+    const int lane = threadIdx.x;
+    const uint8_t *rv = S.vr + R_OFF;
+    // y' = y - A r: every A column times its r nibble, XOR-reduced over lanes
+    if (lane < 4 * NG) S.Ar[lane] = 0;
+    __syncthreads();
+    {
+        uint32_t part[4][NG];
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) part[d][g] = 0;
+#pragma unroll
+        for (int s = 0; s < NCL; ++s) {
+            const int c = col_of(s, lane);
+            const uint32_t rc = c < KO ? mvec::nib(rv, c) : 0u;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                uint32_t p0 = cols[s][0][g], p1 = cols[s][1][g], p2 = cols[s][2][g], p3 = cols[s][3][g];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t m = gf16::bitmask(rc, b);
+                    part[0][g] ^= p0 & m; part[1][g] ^= p1 & m; part[2][g] ^= p2 & m; part[3][g] ^= p3 & m;
+                    bs_mul_a(p0, p1, p2, p3);
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (part[d][g]) atomicXor(&S.Ar[d * NG + g], part[d][g]);
+    }
+    __syncthreads();
+    if (lane == Y_LANE) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) cols[Y_SET][d][g] ^= S.Ar[d * NG + g];
+    }
+    uint32_t used[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) used[g] = 0;
+    int rank = 0;
+    // pivots in column order; set NCL-1 (columns < SHIFT) first, then sets 0..
+#pragma unroll
+    for (int ord = 0; ord < NCL; ++ord) {
+        const int ps = ord == 0 ? NCL - 1 : ord - 1;  // set holding these pivot columns
+        const int nl = ps == NCL - 1 ? SHIFT : 64;
+        for (int pl = 0; pl < nl; ++pl) {
+            const int pc = col_of(ps, pl);
+            if (pc >= KO) break;
+            // candidate rows: nonzero entry in column pc and not yet used
+            int pr = -1;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                uint32_t nz = (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~used[g];
+                nz = __builtin_amdgcn_readlane(nz, pl);
+                if (pr < 0 && nz) pr = 32 * g + __builtin_ctz(nz);
+            }
+            if (pr < 0) {
+                if (lane == 0) S.prow[pc] = -1;
+                continue;
+            }
+            const int pg = pr >> 5, pb = pr & 31;
+            // broadcast the pivot column; flip plane 0 at row pr so that the
+            // update also normalises the pivot row (new A[pr][c] = A[pr][c]/p)
+            uint32_t f[4][NG];
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
+            uint32_t p = 0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g == pg) {
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) p |= ((f[d][g] >> pb) & 1u) << d;
+                    f[0][g] ^= 1u << pb;
+                }
+            const uint32_t inv = uni(gf16::inv(p));
+            // update the sets that can still hold columns > pc, and the y' set
+#pragma unroll
+            for (int s = 0; s < NCL; ++s) {
+                // set NCL-1 holds columns < SHIFT: done once the pivots pass it,
+                // except when it holds y'
+                // sets whose columns all precede pc need no update, unless
+                // they hold y'
+                if (s != Y_SET && set_rank(s) < ord) continue;
+                uint32_t sw[4];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    uint32_t wsel = cols[s][d][0];
+#pragma unroll
+                    for (int g = 1; g < NG; ++g)
+                        if (g == pg) wsel = cols[s][d][g];
+                    sw[d] = (wsel >> pb) & 1u;
+                }
+                const uint32_t sn = sw[0] | (sw[1] << 1) | (sw[2] << 2) | (sw[3] << 3);
+                const uint32_t gn = tmul(S, sn, inv);
+                const uint32_t m0 = gf16::bitmask(gn, 0), m1 = gf16::bitmask(gn, 1);
+                const uint32_t m2 = gf16::bitmask(gn, 2), m3 = gf16::bitmask(gn, 3);
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    // gn * f on the bit-planes: sum_b m_b (a^b f)
+                    uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
+                    uint32_t o0 = a0 & m0, o1 = a1 & m0, o2 = a2 & m0, o3 = a3 & m0;
+                    bs_mul_a(a0, a1, a2, a3);
+                    o0 ^= a0 & m1; o1 ^= a1 & m1; o2 ^= a2 & m1; o3 ^= a3 & m1;
+                    bs_mul_a(a0, a1, a2, a3);
+                    o0 ^= a0 & m2; o1 ^= a1 & m2; o2 ^= a2 & m2; o3 ^= a3 & m2;
+                    bs_mul_a(a0, a1, a2, a3);
+                    o0 ^= a0 & m3; o1 ^= a1 & m3; o2 ^= a2 & m3; o3 ^= a3 & m3;
+                    cols[s][0][g] ^= o0; cols[s][1][g] ^= o1; cols[s][2][g] ^= o2; cols[s][3][g] ^= o3;
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g == pg) used[g] |= 1u << pb;
+            if (lane == 0) S.prow[pc] = (int16_t)pr;
+            ++rank;
+        }
+    }
+    // x = r, plus y'[pivot row] on the pivot columns
+    if (lane == Y_LANE) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) S.yplanes[d * NG + g] = cols[Y_SET][d][g];
+    }
+    __syncthreads();
+    const int ok = rank == M;
+    for (int c = lane; c < KO; c += 64) {
+        uint32_t xv = mvec::nib(rv, c);
+        const int pr = S.prow[c];
+        if (ok && pr >= 0) {
+            uint32_t yn = 0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) yn |= ((S.yplanes[d * NG + (pr >> 5)] >> (pr & 31)) & 1u) << d;
+            xv ^= yn;
+        }
+        S.x[c] = (uint8_t)xv;
+    }
+    __syncthreads();
+    return ok;
+}
+
+template <class Acc>
+__global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n) {
+    __shared__ Shared S;
+    const int lane = threadIdx.x;
+    SignItem it = acc(blockIdx.x);
+    const uint8_t *Ob = it.esk + SK;
+    const uint8_t *P1 = Ob + O_BYTES;
+    const uint8_t *L = P1 + P1_BYTES;
+    for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
+    for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
+    for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
+    int ok = 0, ctr = 0;
+    for (;; ++ctr) {
+        __syncthreads();
+        for (int c = lane; c < V; c += 64) {
+            uint64_t v = 0;
+            for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(S.vr + j * V_BYTES, c) << (4 * j);
+            S.scol[c] = v;
+        }
+        for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
+        __syncthreads();
+        // M_i = V_i L (sign.cu:256-330)
+        for (int item = lane; item < ((O * MW + 63) / 64) * 64; item += 64) {
+            const bool act = item < O * MW;
+            const int x = act ? item / MW : 0, w = act ? item % MW : 0;
+            const int nv = mvec::word_valid(w);
+            uint32_t a[K];
+#pragma unroll
+            for (int i = 0; i < K; ++i) a[i] = 0;
+            for (int r = 0; r < V; ++r) {
+                const uint64_t sc = uni64(S.scol[r]);
+                if (act) {
+                    gf16::Multiples ml(mvec::load_word(L + (size_t)(r * O + x) * MB + 4 * w, nv));
+#pragma unroll
+                    for (int i = 0; i < K; ++i) {
+                        const uint32_t si = (uint32_t)(sc >> (4 * i));
+                        a[i] = gf16::mac(a[i], ml.x0, ml.x1, ml.x2, ml.x3, gf16::bitmask(si, 0),
+                                         gf16::bitmask(si, 1), gf16::bitmask(si, 2),
+                                         gf16::bitmask(si, 3));
+                    }
+                }
+            }
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < K; ++i) S.Mv[(i * O + x) * MW + w] = a[i];
+            }
+        }
+        // U = V^T P1 V per pair (sign.cu:396-437)
+        quad_form<false>(P1, nullptr, nullptr, S.scol, S.U);
+        __syncthreads();
+        wave_y(S, S.t);
+        uint32_t cols[NCL][4][NG];
+        build_columns(S, cols);
+        ok = solve(S, cols);
+        if (ok || ctr == MAX_ATTEMPTS - 1) break;
+        // rare path: next attempt's V || r = SHAKE256(digest || salt ||
+        // seed_sk || ctr) (sign.cu:251-252), hashed by the whole wave
+        for (int b = lane; b < DIGEST + SK; b += 64) S.hbuf[b] = it.scratch[SCR_DIGEST + b];
+        for (int b = lane; b < SK; b += 64) S.hbuf[DIGEST + SK + b] = it.esk[b];
+        if (lane == 0) S.hbuf[DIGEST + 2 * SK] = (uint8_t)(ctr + 1);
+        __syncthreads();
+        keccak::wave_shake256(S.vr, VR_BYTES, S.hbuf, DIGEST + 2 * SK + 1);
+    }
+    if (lane == 0) it.scratch[SCR_ATTEMPTS] = ok ? (uint8_t)(ctr + 1) : 0;
+    __syncthreads();
+    // s_i = (V_i + O x_i || x_i), salt (sign.cu:147-191, 472-477)
+    for (int b = lane; b < SIG_S_BYTES; b += 64) {
+        uint32_t out = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = 2 * b + h, i = e / N, t = e % N;
+            uint32_t val;
+            if (t < V) {
+                uint32_t a = 0;
+#pragma unroll
+                for (int kk = 0; kk < O; ++kk) a ^= tmul(S, mvec::nib(Ob, t * O + kk), S.x[i * O + kk]);
+                val = mvec::nib(S.vr + i * V_BYTES, t) ^ a;
+            } else {
+                val = S.x[i * O + (t - V)];
+            }
+            out |= val << (4 * h);
+        }
+        it.sig[b] = (uint8_t)out;
+    }
+    for (int b = lane; b < SK; b += 64) it.sig[SIG_S_BYTES + b] = it.scratch[SCR_SALT + b];
+}
+
+// Verification, one wave per item (verify.cu:9-211).
+template <class Acc>
+__global__ void __launch_bounds__(64, 3) k_verify_wave(Acc acc, size_t n) {
+    __shared__ uint64_t scol[N];
+    __shared__ uint32_t U[PAIRS * MW];
+    __shared__ uint32_t yu[UW];
+    __shared__ uint8_t tb[MB + 4];
+    __shared__ uint8_t hbuf[DIGEST + SK];
+    const int lane = threadIdx.x;
+    VerifyItem it = acc(blockIdx.x);
+    const uint8_t *P1 = it.epk, *P2 = P1 + P1_BYTES, *P3 = P2 + P2_BYTES;
+    for (int c = lane; c < N; c += 64) {
+        uint64_t v = 0;
+        for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(it.sig, j * N + c) << (4 * j);
+        scol[c] = v;
+    }
+    for (int e = lane; e < PAIRS * MW; e += 64) U[e] = 0;
+    if (it.t) {
+        for (int b = lane; b < MB; b += 64) tb[b] = it.t[b];
+    } else {
+        // digest = SHAKE256(msg), t = SHAKE256(digest || salt) (verify.cu:46-48)
+        keccak::wave_shake256(hbuf, DIGEST, it.msg, it.mlen);
+        for (int b = lane; b < SK; b += 64) hbuf[DIGEST + b] = it.sig[SIG_S_BYTES + b];
+        __syncthreads();
+        keccak::wave_shake256(tb, MB, hbuf, DIGEST + SK);
+    }
+    __syncthreads();
+    quad_form<true>(P1, P2, P3, scol, U);
+    __syncthreads();
+    if (lane < UW) {
+        uint32_t a = 0;
+        for (int i = 0; i < K; ++i)
+            for (int j = i; j < K; ++j) {
+                const int ell = ell_of(i, j), ws = ell >> 3, ns = 4 * (ell & 7);
+                const uint32_t *src = &U[pair_index(i, j) * MW];
+                const int hi = lane - ws, lo = lane - ws - 1;
+                uint32_t h = (hi >= 0 && hi < MW) ? src[hi] : 0u;
+                uint32_t l = (lo >= 0 && lo < MW) ? src[lo] : 0u;
+                a ^= ns ? __builtin_amdgcn_alignbit(h, l, 32 - ns) : h;
+            }
+        yu[lane] = a;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        uint32_t a[UW], y[MW];
+#pragma unroll
+        for (int q = 0; q < UW; ++q) a[q] = yu[q];
+        reduce_unreduced(a, y);
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+            const int nv = mvec::word_valid(w);
+            uint32_t tw = 0;
+            for (int b = 0; b < nv; ++b) tw |= (uint32_t)tb[4 * w + b] << (8 * b);
+            eq &= (y[w] == tw);
+        }
+        it.result[0] = eq ? 0 : 1;
+    }
+}
+
+}  // namespace wv
+}  // namespace mayo

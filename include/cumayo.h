@@ -126,7 +126,9 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
 /* The same in two stages, selected by the bit mask `stages`: 1 = the
  * hashing pass (digest, salt, target, vinegar vector of attempt 0, one lane
  * per item), 2 = the linear-algebra pass (one workgroup per item). Stage 2
- * reads what stage 1 left in the workspace; 3 runs both. */
+ * reads what stage 1 left in the workspace; 3 runs both. Adding 4 runs
+ * stage 2 with the workgroup-per-item kernels instead of the default
+ * wave-per-item kernels (same results; kept as the comparison form). */
 int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
                              const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
                              uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
