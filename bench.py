@@ -9,9 +9,11 @@ from SHAKE streams and are in HBM before the timed region.
 
 Prints one JSON line. ``value`` is signatures per second; verification
 throughput is reported beside it. ``roofline`` covers the signing
-linear-algebra kernel (the dominant kernel), timed with HIP events on the
-stream it runs on. ``cpu_baseline`` times the CPU oracle (the restatement of
-the reference algorithm) on a bounded sample of the same workload.
+linear-algebra pass (the dominant kernels), timed with HIP events on the
+stream it runs on. ``serve`` drives the continuous-batching server with
+open-loop Poisson arrivals at a fraction of the batch throughput and reports
+latency percentiles. ``cpu_baseline`` times the CPU oracle (the restatement
+of the reference algorithm) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -65,6 +67,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="signatures the CPU baseline signs (bounded sample)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="length of each open-loop serve run (0 skips the serve runs)")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="offered Poisson rate as a fraction of the measured batch throughput")
+    ap.add_argument("--serve-batch", type=int, default=16384, help="requests per serve slot")
+    ap.add_argument("--serve-slots", type=int, default=3, help="serve batches in flight")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -181,7 +189,9 @@ def main():
                          "max_attempts": int(attempts.max())},
         "roofline": {
             "bound": "valu",
-            "kernel": "k_sign_wave (signing linear algebra, one wavefront per signature)",
+            "kernel": "signing linear-algebra pass: k_sign_prep (M, U, y) + k_sign_solve "
+                      "(elimination) + k_sign_wave retry pass, one wavefront per signature; "
+                      "achieved and kernel_ms cover the three launches together",
             "achieved": round(achieved_tmacs, 3),
             "peak": round(PEAK_GF16_TMACS, 1),
             "unit": "T GF16-MAC/s",
@@ -193,6 +203,34 @@ def main():
             "traffic": None,
         },
     }
+
+    if args.serve_seconds > 0:
+        # Continuous batching under open-loop Poisson arrivals drawn from the
+        # same workload (request i: message i, key i mod keys; verification
+        # requests carry the signature just made for message i). Latency is
+        # arrival to result in host memory, including both PCIe copies.
+        from cumayo_amd.serve import Server
+        serve = {}
+        mixed_rate = 1.0 / (0.5 / sign_per_s + 0.5 / verify_per_s)
+        with Server(eng, d_esk, d_epk, max_batch=args.serve_batch, n_slots=args.serve_slots,
+                    slot_msg_bytes=args.serve_batch * 1024) as srv:
+            for name, frac_v, batch_rate in (("sign", 0.0, sign_per_s), ("mixed_50_50", 0.5, mixed_rate)):
+                rate = args.serve_load * batch_rate
+                print(f"[bench] serve {name}: Poisson {rate:.0f}/s for {args.serve_seconds} s",
+                      file=sys.stderr, flush=True)
+                rep = srv.run_poisson(rate, args.serve_seconds, frac_v, 17, flat, offs, lens, kidx, sigs)
+                serve[name] = {"offered_per_s": round(rep["offered_per_s"], 1),
+                               "achieved_per_s": round(rep["achieved_per_s"], 1),
+                               "offered_frac_of_batch": args.serve_load,
+                               "p50_ms": round(rep["p50_ms"], 3), "p90_ms": round(rep["p90_ms"], 3),
+                               "p99_ms": round(rep["p99_ms"], 3), "max_ms": round(rep["max_ms"], 3),
+                               "completed": rep["completed"], "refused": rep["refused"]}
+            st = srv.stats()
+        serve["slots"] = {"max_batch": args.serve_batch, "n_slots": args.serve_slots,
+                          "batches": st["batches"], "overlapped_launches": st["overlapped_launches"]}
+        out["serve"] = serve
+        out["serve_p50_ms"] = serve["sign"]["p50_ms"]
+        out["serve_p99_ms"] = serve["sign"]["p99_ms"]
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O

@@ -99,7 +99,7 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
-    wv::k_sign_wave<PitchedSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+    wv::k_sign_wave<PitchedSign><<<(unsigned)n, 64, 0, stream>>>(acc, n, 0);
     note_launch();
 }
 
@@ -279,10 +279,15 @@ int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const 
         e = hipGetLastError();
     }
     if (e == hipSuccess && (stages & 2)) {
-        if (stages & 4)
+        if (stages & 4) {
             k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
-        else
-            wv::k_sign_wave<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+        } else {
+            // attempt 0 in two kernels tuned for their register needs, then
+            // the rank failures (about 1 in 4,000) through the full loop
+            wv::k_sign_prep<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+            wv::k_sign_solve<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+            wv::k_sign_wave<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1);
+        }
         e = hipGetLastError();
     }
     note(e);
@@ -328,6 +333,52 @@ int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32
                         hipStream_t stream) {
     return cumayo_verify_table_stages(epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs,
                                       results, n, workspace, stream, 3);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Continuous-batching server (C ABI)
+// ---------------------------------------------------------------------------
+#include "serve.hpp"
+
+extern "C" {
+
+void *cumayo_serve_create(const cumayo_serve_config *cfg) {
+    if (!cfg) return nullptr;
+    auto *s = new serve::Server(*cfg);
+    if (s->error()) {
+        note((hipError_t)s->error());
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void cumayo_serve_destroy(void *server) { delete static_cast<serve::Server *>(server); }
+
+size_t cumayo_serve_submit(void *server, const cumayo_request *reqs, size_t n) {
+    return server ? static_cast<serve::Server *>(server)->submit(reqs, n) : 0;
+}
+
+size_t cumayo_serve_poll(void *server, cumayo_completion *out, size_t max) {
+    return server ? static_cast<serve::Server *>(server)->poll(out, max) : 0;
+}
+
+void cumayo_serve_get_stats(void *server, cumayo_serve_stats *out) {
+    if (server && out) static_cast<serve::Server *>(server)->stats(out);
+}
+
+int64_t cumayo_serve_clock_ns(void) { return serve::now_ns(); }
+
+int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
+                             double verify_fraction, uint64_t seed, const uint8_t *msg_pool,
+                             const int64_t *msg_off, const int32_t *msg_len, const int32_t *msg_key,
+                             const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out) {
+    if (!server || !out) return (int)hipErrorInvalidValue;
+    return serve::run_poisson(*static_cast<serve::Server *>(server), rate_per_s, duration_s,
+                              verify_fraction, seed, msg_pool, msg_off, msg_len, msg_key, sig_pool,
+                              pool_n, out);
 }
 
 }  // extern "C"

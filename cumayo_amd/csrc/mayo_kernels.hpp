@@ -34,7 +34,12 @@ constexpr int SCR_T = DIGEST + SK;
 constexpr int SCR_VR = DIGEST + SK + MB;
 constexpr int SCR_ATTEMPTS = SCR_VR + VR_BYTES;  // attempts used (0 = all failed)
 constexpr int SIGN_SCRATCH = SCR_ATTEMPTS + 1;
-constexpr int SIGN_SCRATCH_STRIDE = (SIGN_SCRATCH + 63) / 64 * 64;
+// Batch form only: the M products and y of attempt 0, handed from the
+// product kernel to the solve kernel.
+constexpr int SCR_MV = (SIGN_SCRATCH + 63) / 64 * 64;
+constexpr int SCR_Y = SCR_MV + K * O * MW * 4;
+constexpr int SIGN_SCRATCH_STRIDE = (SCR_Y + MW * 4 + 63) / 64 * 64;
+constexpr uint8_t ATTEMPT_PENDING = 0xFF;   // rank failure left for the retry pass
 // Scratch per verifying item: t.
 constexpr int VERIFY_SCRATCH_STRIDE = (MB + 15) / 16 * 16;
 

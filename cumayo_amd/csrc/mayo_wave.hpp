@@ -31,6 +31,9 @@ constexpr int NCL = (NCOL + 63) / 64;          // columns held per lane
 constexpr int OVW = (PAIRS + 7) / 8 + 1;       // overflow words of an unreduced sum
 constexpr int UW = MW + OVW;                   // words of an unreduced sum
 constexpr int RT = 2;                          // quad-form rows per chunk per lane
+// waves per SIMD the quadratic-form kernels are compiled for: the pair
+// accumulators fit 128 VGPRs up to MAYO_2's sizes, the larger sets use 2
+constexpr int QF_OCC = (M <= 80) ? 4 : 2;
 // column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
 // go to the last set so that only one set stays active for most pivots
 constexpr int SHIFT = NCOL - 64 * (NCL - 1);
@@ -77,6 +80,7 @@ struct Shared {
     uint32_t Ar[4 * NG];         // A r, bitsliced
     uint32_t yplanes[4 * NG];    // final y' column, bitsliced
     int16_t prow[KO];            // pivot row of each column, -1 if free
+    uint8_t pval[KO];            // product of the pivot values before each column
     uint8_t x[KO];
     uint8_t mul[256];            // GF(16) product table, a*16 + b
     uint8_t hbuf[DIGEST + 2 * SK + 1];  // hash input staging
@@ -322,29 +326,27 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
 #pragma unroll
     for (int g = 0; g < NG; ++g) used[g] = 0;
     int rank = 0;
-    // pivots in column order; set NCL-1 (columns < SHIFT) first, then sets 0..
+    uint32_t prefix = 1;
+    // pivots in column order: set NCL-1 (columns < SHIFT) first, then 0, 1, ...
 #pragma unroll
     for (int ord = 0; ord < NCL; ++ord) {
         const int ps = ord == 0 ? NCL - 1 : ord - 1;  // set holding these pivot columns
         const int nl = ps == NCL - 1 ? SHIFT : 64;
+#pragma unroll 1
         for (int pl = 0; pl < nl; ++pl) {
             const int pc = col_of(ps, pl);
             if (pc >= KO) break;
-            // candidate rows: nonzero entry in column pc and not yet used
+            // pivot row: lowest unused row with a nonzero entry in column pc
             int pr = -1;
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                uint32_t nz = (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~used[g];
-                nz = __builtin_amdgcn_readlane(nz, pl);
+                const uint32_t nz = __builtin_amdgcn_readlane(
+                    (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~used[g], pl);
                 if (pr < 0 && nz) pr = 32 * g + __builtin_ctz(nz);
             }
-            if (pr < 0) {
-                if (lane == 0) S.prow[pc] = -1;
-                continue;
-            }
-            const int pg = pr >> 5, pb = pr & 31;
-            // broadcast the pivot column; flip plane 0 at row pr so that the
-            // update also normalises the pivot row (new A[pr][c] = A[pr][c]/p)
+            const bool has = pr >= 0;
+            const int prr = has ? pr : 0, pg = prr >> 5, pb = prr & 31;
+            // broadcast the pivot column f; p = f[pr]
             uint32_t f[4][NG];
 #pragma unroll
             for (int d = 0; d < 4; ++d)
@@ -356,49 +358,65 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
                 if (g == pg) {
 #pragma unroll
                     for (int d = 0; d < 4; ++d) p |= ((f[d][g] >> pb) & 1u) << d;
-                    f[0][g] ^= 1u << pb;
                 }
-            const uint32_t inv = uni(gf16::inv(p));
-            // update the sets that can still hold columns > pc, and the y' set
+            // Fraction-free step, no inverse: every column c becomes
+            // p * c + c[pr] * f', with f' = f except f'[pr] = p + 1. Rows
+            // a != pr: p c[a] + c[pr] f[a] (column pc turns zero there); row
+            // pr keeps c[pr]. A free column (no pivot) takes p = 1, f' = 0.
+#pragma unroll
+            for (int g = 0; g < NG; ++g) f[0][g] ^= (has && g == pg) ? (1u << pb) : 0u;
+            const uint32_t pu = has ? p : 1u;
+            const uint32_t pm0 = gf16::bitmask(pu, 0), pm1 = gf16::bitmask(pu, 1);
+            const uint32_t pm2 = gf16::bitmask(pu, 2), pm3 = gf16::bitmask(pu, 3);
+            const uint32_t fz = has ? 0xFFFFFFFFu : 0u;
 #pragma unroll
             for (int s = 0; s < NCL; ++s) {
-                // set NCL-1 holds columns < SHIFT: done once the pivots pass it,
-                // except when it holds y'
-                // sets whose columns all precede pc need no update, unless
-                // they hold y'
+                // sets whose columns all precede pc need no update (their
+                // pivot columns stay scaled unit vectors; free columns do not
+                // enter x), unless they hold y'
                 if (s != Y_SET && set_rank(s) < ord) continue;
-                uint32_t sw[4];
+                uint32_t m[4];
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     uint32_t wsel = cols[s][d][0];
 #pragma unroll
-                    for (int g = 1; g < NG; ++g)
-                        if (g == pg) wsel = cols[s][d][g];
-                    sw[d] = (wsel >> pb) & 1u;
+                    for (int g = 1; g < NG; ++g) wsel = (g == pg) ? cols[s][d][g] : wsel;
+                    m[d] = (uint32_t)__builtin_amdgcn_sbfe((int)wsel, (uint32_t)pb, 1u) & fz;
                 }
-                const uint32_t sn = sw[0] | (sw[1] << 1) | (sw[2] << 2) | (sw[3] << 3);
-                const uint32_t gn = tmul(S, sn, inv);
-                const uint32_t m0 = gf16::bitmask(gn, 0), m1 = gf16::bitmask(gn, 1);
-                const uint32_t m2 = gf16::bitmask(gn, 2), m3 = gf16::bitmask(gn, 3);
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    // gn * f on the bit-planes: sum_b m_b (a^b f)
+                    // p * c: multiples of the lane's own column, uniform masks
+                    uint32_t c0 = cols[s][0][g], c1 = cols[s][1][g], c2 = cols[s][2][g], c3 = cols[s][3][g];
+                    uint32_t o0 = c0 & pm0, o1 = c1 & pm0, o2 = c2 & pm0, o3 = c3 & pm0;
+                    bs_mul_a(c0, c1, c2, c3);
+                    o0 = gf16::axm(o0, c0, pm1); o1 = gf16::axm(o1, c1, pm1);
+                    o2 = gf16::axm(o2, c2, pm1); o3 = gf16::axm(o3, c3, pm1);
+                    bs_mul_a(c0, c1, c2, c3);
+                    o0 = gf16::axm(o0, c0, pm2); o1 = gf16::axm(o1, c1, pm2);
+                    o2 = gf16::axm(o2, c2, pm2); o3 = gf16::axm(o3, c3, pm2);
+                    bs_mul_a(c0, c1, c2, c3);
+                    o0 = gf16::axm(o0, c0, pm3); o1 = gf16::axm(o1, c1, pm3);
+                    o2 = gf16::axm(o2, c2, pm3); o3 = gf16::axm(o3, c3, pm3);
+                    // + c[pr] * f': uniform multiples of f', per-lane masks
                     uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
-                    uint32_t o0 = a0 & m0, o1 = a1 & m0, o2 = a2 & m0, o3 = a3 & m0;
-                    bs_mul_a(a0, a1, a2, a3);
-                    o0 ^= a0 & m1; o1 ^= a1 & m1; o2 ^= a2 & m1; o3 ^= a3 & m1;
-                    bs_mul_a(a0, a1, a2, a3);
-                    o0 ^= a0 & m2; o1 ^= a1 & m2; o2 ^= a2 & m2; o3 ^= a3 & m2;
-                    bs_mul_a(a0, a1, a2, a3);
-                    o0 ^= a0 & m3; o1 ^= a1 & m3; o2 ^= a2 & m3; o3 ^= a3 & m3;
-                    cols[s][0][g] ^= o0; cols[s][1][g] ^= o1; cols[s][2][g] ^= o2; cols[s][3][g] ^= o3;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        o0 = gf16::axm(o0, a0, m[b]); o1 = gf16::axm(o1, a1, m[b]);
+                        o2 = gf16::axm(o2, a2, m[b]); o3 = gf16::axm(o3, a3, m[b]);
+                        bs_mul_a(a0, a1, a2, a3);
+                    }
+                    cols[s][0][g] = o0; cols[s][1][g] = o1; cols[s][2][g] = o2; cols[s][3][g] = o3;
                 }
             }
+            // product of the earlier pivots: z at this column is later
+            // y'[pr] * prefix / (product of all pivots)
+            S.pval[pc] = (uint8_t)prefix;
+            if (has) prefix = gf16::mul(prefix, p);
 #pragma unroll
             for (int g = 0; g < NG; ++g)
-                if (g == pg) used[g] |= 1u << pb;
-            if (lane == 0) S.prow[pc] = (int16_t)pr;
-            ++rank;
+                if (has && g == pg) used[g] |= 1u << pb;
+            rank += has ? 1 : 0;
+            S.prow[pc] = (int16_t)pr;  // same value from every lane
         }
     }
     // x = r, plus y'[pivot row] on the pivot columns
@@ -410,6 +428,7 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
     }
     __syncthreads();
     const int ok = rank == M;
+    const uint32_t tinv = gf16::inv(prefix);
     for (int c = lane; c < KO; c += 64) {
         uint32_t xv = mvec::nib(rv, c);
         const int pr = S.prow[c];
@@ -417,7 +436,7 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
             uint32_t yn = 0;
 #pragma unroll
             for (int d = 0; d < 4; ++d) yn |= ((S.yplanes[d * NG + (pr >> 5)] >> (pr & 31)) & 1u) << d;
-            xv ^= yn;
+            xv ^= tmul(S, tmul(S, yn, S.pval[c]), tinv);
         }
         S.x[c] = (uint8_t)xv;
     }
@@ -425,72 +444,48 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
     return ok;
 }
 
-template <class Acc>
-__global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n) {
-    __shared__ Shared S;
+// M_i = V_i L (sign.cu:256-330): lanes = (column x, word w) items, r uniform.
+__device__ __forceinline__ void wave_M(Shared &S, const uint8_t *L) {
     const int lane = threadIdx.x;
-    SignItem it = acc(blockIdx.x);
-    const uint8_t *Ob = it.esk + SK;
-    const uint8_t *P1 = Ob + O_BYTES;
-    const uint8_t *L = P1 + P1_BYTES;
-    for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
-    for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
-    for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
-    int ok = 0, ctr = 0;
-    for (;; ++ctr) {
-        __syncthreads();
-        for (int c = lane; c < V; c += 64) {
-            uint64_t v = 0;
-            for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(S.vr + j * V_BYTES, c) << (4 * j);
-            S.scol[c] = v;
-        }
-        for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
-        __syncthreads();
-        // M_i = V_i L (sign.cu:256-330)
-        for (int item = lane; item < ((O * MW + 63) / 64) * 64; item += 64) {
-            const bool act = item < O * MW;
-            const int x = act ? item / MW : 0, w = act ? item % MW : 0;
-            const int nv = mvec::word_valid(w);
-            uint32_t a[K];
+    for (int item = lane; item < ((O * MW + 63) / 64) * 64; item += 64) {
+        const bool act = item < O * MW;
+        const int x = act ? item / MW : 0, w = act ? item % MW : 0;
+        const int nv = mvec::word_valid(w);
+        uint32_t a[K];
 #pragma unroll
-            for (int i = 0; i < K; ++i) a[i] = 0;
-            for (int r = 0; r < V; ++r) {
-                const uint64_t sc = uni64(S.scol[r]);
-                if (act) {
-                    gf16::Multiples ml(mvec::load_word(L + (size_t)(r * O + x) * MB + 4 * w, nv));
+        for (int i = 0; i < K; ++i) a[i] = 0;
+        for (int r = 0; r < V; ++r) {
+            const uint64_t sc = uni64(S.scol[r]);
+            if (act) {
+                gf16::Multiples ml(mvec::load_word(L + (size_t)(r * O + x) * MB + 4 * w, nv));
 #pragma unroll
-                    for (int i = 0; i < K; ++i) {
-                        const uint32_t si = (uint32_t)(sc >> (4 * i));
-                        a[i] = gf16::mac(a[i], ml.x0, ml.x1, ml.x2, ml.x3, gf16::bitmask(si, 0),
-                                         gf16::bitmask(si, 1), gf16::bitmask(si, 2),
-                                         gf16::bitmask(si, 3));
-                    }
+                for (int i = 0; i < K; ++i) {
+                    const uint32_t si = (uint32_t)(sc >> (4 * i));
+                    a[i] = gf16::mac(a[i], ml.x0, ml.x1, ml.x2, ml.x3, gf16::bitmask(si, 0),
+                                     gf16::bitmask(si, 1), gf16::bitmask(si, 2),
+                                     gf16::bitmask(si, 3));
                 }
             }
-            if (act) {
-#pragma unroll
-                for (int i = 0; i < K; ++i) S.Mv[(i * O + x) * MW + w] = a[i];
-            }
         }
-        // U = V^T P1 V per pair (sign.cu:396-437)
-        quad_form<false>(P1, nullptr, nullptr, S.scol, S.U);
-        __syncthreads();
-        wave_y(S, S.t);
-        uint32_t cols[NCL][4][NG];
-        build_columns(S, cols);
-        ok = solve(S, cols);
-        if (ok || ctr == MAX_ATTEMPTS - 1) break;
-        // rare path: next attempt's V || r = SHAKE256(digest || salt ||
-        // seed_sk || ctr) (sign.cu:251-252), hashed by the whole wave
-        for (int b = lane; b < DIGEST + SK; b += 64) S.hbuf[b] = it.scratch[SCR_DIGEST + b];
-        for (int b = lane; b < SK; b += 64) S.hbuf[DIGEST + SK + b] = it.esk[b];
-        if (lane == 0) S.hbuf[DIGEST + 2 * SK] = (uint8_t)(ctr + 1);
-        __syncthreads();
-        keccak::wave_shake256(S.vr, VR_BYTES, S.hbuf, DIGEST + 2 * SK + 1);
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) S.Mv[(i * O + x) * MW + w] = a[i];
+        }
     }
-    if (lane == 0) it.scratch[SCR_ATTEMPTS] = ok ? (uint8_t)(ctr + 1) : 0;
-    __syncthreads();
-    // s_i = (V_i + O x_i || x_i), salt (sign.cu:147-191, 472-477)
+}
+
+__device__ __forceinline__ void wave_scol(Shared &S) {
+    for (int c = threadIdx.x; c < V; c += 64) {
+        uint64_t v = 0;
+        for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(S.vr + j * V_BYTES, c) << (4 * j);
+        S.scol[c] = v;
+    }
+}
+
+// s_i = (V_i + O x_i || x_i) and the salt (sign.cu:147-191, 472-477)
+__device__ __forceinline__ void wave_write_sig(const Shared &S, const SignItem &it) {
+    const int lane = threadIdx.x;
+    const uint8_t *Ob = it.esk + SK;
     for (int b = lane; b < SIG_S_BYTES; b += 64) {
         uint32_t out = 0;
 #pragma unroll
@@ -512,9 +507,107 @@ __global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n) {
     for (int b = lane; b < SK; b += 64) it.sig[SIG_S_BYTES + b] = it.scratch[SCR_SALT + b];
 }
 
+// The whole signing loop in one wave (sign.cu:244-478). With retry_only set
+// it handles only items the solve kernel left pending, from attempt 1 on.
+template <class Acc>
+__global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n, int retry_only) {
+    __shared__ Shared S;
+    const int lane = threadIdx.x;
+    SignItem it = acc(blockIdx.x);
+    if (retry_only && it.scratch[SCR_ATTEMPTS] != ATTEMPT_PENDING) return;
+    const uint8_t *P1 = it.esk + SK + O_BYTES;
+    const uint8_t *L = P1 + P1_BYTES;
+    for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
+    for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
+    int ctr = retry_only ? 1 : 0;
+    // V || r of the first attempt handled here: from the hashing pass for
+    // attempt 0, re-derived for attempt ctr >= 1 (sign.cu:251-252)
+    auto derive_vr = [&](int c) {
+        for (int b = lane; b < DIGEST + SK; b += 64) S.hbuf[b] = it.scratch[SCR_DIGEST + b];
+        for (int b = lane; b < SK; b += 64) S.hbuf[DIGEST + SK + b] = it.esk[b];
+        if (lane == 0) S.hbuf[DIGEST + 2 * SK] = (uint8_t)c;
+        __syncthreads();
+        keccak::wave_shake256(S.vr, VR_BYTES, S.hbuf, DIGEST + 2 * SK + 1);
+    };
+    if (ctr == 0) {
+        for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
+    } else {
+        derive_vr(ctr);
+    }
+    int ok = 0;
+    for (;; ++ctr) {
+        __syncthreads();
+        wave_scol(S);
+        for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
+        __syncthreads();
+        wave_M(S, L);
+        quad_form<false>(P1, nullptr, nullptr, S.scol, S.U);
+        __syncthreads();
+        wave_y(S, S.t);
+        uint32_t cols[NCL][4][NG];
+        build_columns(S, cols);
+        ok = solve(S, cols);
+        if (ok || ctr == MAX_ATTEMPTS - 1) break;
+        derive_vr(ctr + 1);
+    }
+    if (lane == 0) it.scratch[SCR_ATTEMPTS] = ok ? (uint8_t)(ctr + 1) : 0;
+    __syncthreads();
+    wave_write_sig(S, it);
+}
+
+// Batch form, attempt 0, part 1: M, U and y, handed over through the
+// workspace. Register-heavy (the pair accumulators of the quadratic form).
+template <class Acc>
+__global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
+    __shared__ Shared S;
+    const int lane = threadIdx.x;
+    SignItem it = acc(blockIdx.x);
+    const uint8_t *P1 = it.esk + SK + O_BYTES;
+    const uint8_t *L = P1 + P1_BYTES;
+    for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
+    for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
+    __syncthreads();
+    wave_scol(S);
+    for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
+    __syncthreads();
+    wave_M(S, L);
+    quad_form<false>(P1, nullptr, nullptr, S.scol, S.U);
+    __syncthreads();
+    wave_y(S, S.t);
+    uint32_t *mv = (uint32_t *)(it.scratch + SCR_MV);
+    for (int e = lane; e < K * O * MW; e += 64) mv[e] = S.Mv[e];
+    uint32_t *yv = (uint32_t *)(it.scratch + SCR_Y);
+    for (int e = lane; e < MW; e += 64) yv[e] = S.y[e];
+}
+
+// Batch form, attempt 0, part 2: columns of [A | y - A r], Gauss-Jordan,
+// signature. Few registers, so many waves per SIMD hide the pivot chain.
+template <class Acc>
+__global__ void __launch_bounds__(64, 8) k_sign_solve(Acc acc, size_t n) {
+    __shared__ Shared S;
+    const int lane = threadIdx.x;
+    SignItem it = acc(blockIdx.x);
+    for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
+    const uint32_t *mv = (const uint32_t *)(it.scratch + SCR_MV);
+    for (int e = lane; e < K * O * MW; e += 64) S.Mv[e] = mv[e];
+    const uint32_t *yv = (const uint32_t *)(it.scratch + SCR_Y);
+    for (int e = lane; e < MW; e += 64) S.y[e] = yv[e];
+    for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
+    __syncthreads();
+    uint32_t cols[NCL][4][NG];
+    build_columns(S, cols);
+    const int ok = solve(S, cols);
+    if (!ok) {
+        if (lane == 0) it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
+        return;
+    }
+    if (lane == 0) it.scratch[SCR_ATTEMPTS] = 1;
+    wave_write_sig(S, it);
+}
+
 // Verification, one wave per item (verify.cu:9-211).
 template <class Acc>
-__global__ void __launch_bounds__(64, 3) k_verify_wave(Acc acc, size_t n) {
+__global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
     __shared__ uint64_t scol[N];
     __shared__ uint32_t U[PAIRS * MW];
     __shared__ uint32_t yu[UW];

@@ -13,8 +13,9 @@
  * cumayo_last_error().
  *
  * Part 2 extends the boundary with the batched key-table form used for
- * shared resident keys and ragged messages; it has no counterpart in the
- * reference.
+ * shared resident keys and ragged messages, and Part 3 with a
+ * continuous-batching request server over it; neither has a counterpart in
+ * the reference.
  */
 #ifndef CUMAYO_H
 #define CUMAYO_H
@@ -109,8 +110,8 @@ typedef struct {
 
 /* Parameter set compiled into this library. */
 void cumayo_get_sizes(cumayo_sizes *out);
-/* Last HIP error recorded by any entry point of this library (0 = none),
- * and its name; reading clears it. */
+/* Last HIP error recorded by any entry point of this library (0 = none);
+ * reading clears it. */
 int cumayo_last_error(void);
 
 /* Batched signing over a table of expanded secret keys (reference esk
@@ -125,10 +126,11 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
 
 /* The same in two stages, selected by the bit mask `stages`: 1 = the
  * hashing pass (digest, salt, target, vinegar vector of attempt 0, one lane
- * per item), 2 = the linear-algebra pass (one workgroup per item). Stage 2
- * reads what stage 1 left in the workspace; 3 runs both. Adding 4 runs
- * stage 2 with the workgroup-per-item kernels instead of the default
- * wave-per-item kernels (same results; kept as the comparison form). */
+ * per item), 2 = the linear-algebra pass (one wavefront per item: M, U and
+ * y, then the solve, then a retry pass over the items whose first attempt
+ * met a singular system). Stage 2 reads what stage 1 left in the workspace;
+ * 3 runs both. Adding 4 runs stage 2 with the workgroup-per-item kernels
+ * instead (same results; kept as the comparison form). */
 int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
                              const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
                              uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
@@ -146,6 +148,79 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
                                const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
                                const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
                                hipStream_t stream, int stages);
+
+/* ---------------- Part 3: continuous-batching server ---------------- */
+
+/* Requests queue on the host; a worker thread admits them into free batch
+ * slots (one HIP stream, pinned staging and device buffers each) as earlier
+ * batches finish, so batches launched while others run overlap on the GPU.
+ * Device memory is bounded by n_slots x (slot_msg_bytes + per-request
+ * buffers for max_batch requests); a request waits until a slot with room
+ * frees up. Results are identical to the batch entry points. No counterpart
+ * in the reference. */
+#define CUMAYO_OP_SIGN 0
+#define CUMAYO_OP_VERIFY 1
+
+typedef struct {
+    int device;
+    const uint8_t *esk_table; size_t esk_stride; /* device; NULL if no signing */
+    const uint8_t *epk_table; size_t epk_stride; /* device; NULL if no verifying */
+    int32_t n_keys;
+    int32_t max_batch;      /* requests per slot */
+    int32_t n_slots;        /* batches in flight */
+    size_t slot_msg_bytes;  /* message bytes per slot */
+    int64_t max_queue;      /* pending requests beyond which submit refuses (0 = no limit) */
+} cumayo_serve_config;
+
+typedef struct {
+    uint8_t op;             /* CUMAYO_OP_SIGN or CUMAYO_OP_VERIFY */
+    int32_t key;            /* row of the key table */
+    const uint8_t *msg;     /* host message */
+    uint32_t len;
+    const uint8_t *sig_in;  /* verify: host signature */
+    uint8_t *sig_out;       /* sign: host output (sig_bytes) or NULL */
+    uint8_t *result_out;    /* host byte or NULL: verify result (0 valid / 1 invalid);
+                               sign: attempts used (0 = all 255 failed) */
+    uint64_t tag;
+    int64_t t_arrival_ns;   /* 0 = now; else the arrival time on cumayo_serve_clock_ns */
+} cumayo_request;
+
+typedef struct {
+    uint64_t tag;
+    int64_t latency_ns;     /* arrival to result on the host */
+    uint8_t status;         /* verify: 0 valid / 1 invalid; sign: 0 ok, 1 all attempts failed;
+                               2 = refused (key out of range, no key table for the op,
+                               or a message longer than slot_msg_bytes) */
+} cumayo_completion;
+
+typedef struct {
+    uint64_t batches, requests, rejected, max_in_flight, overlapped_launches;
+    size_t device_bytes;
+} cumayo_serve_stats;
+
+typedef struct {
+    double offered_per_s, achieved_per_s;
+    double p50_ms, p90_ms, p99_ms, mean_ms, max_ms;
+    uint64_t submitted, completed, refused;
+    double duration_s;
+} cumayo_serve_report;
+
+void *cumayo_serve_create(const cumayo_serve_config *cfg);
+void cumayo_serve_destroy(void *server);
+/* Returns the number of requests accepted (admission stops at max_queue). */
+size_t cumayo_serve_submit(void *server, const cumayo_request *reqs, size_t n);
+size_t cumayo_serve_poll(void *server, cumayo_completion *out, size_t max);
+void cumayo_serve_get_stats(void *server, cumayo_serve_stats *out);
+int64_t cumayo_serve_clock_ns(void);
+/* Open-loop load: Poisson arrivals at rate_per_s for duration_s over a pool
+ * of prepared requests (message i of the pool, key msg_key[i], and for
+ * verification the signature sig_pool + i * sig_bytes); a fraction
+ * verify_fraction of arrivals are verifications. Drains, then reports
+ * latency percentiles (arrival to result). Returns 0 or an error. */
+int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
+                             double verify_fraction, uint64_t seed, const uint8_t *msg_pool,
+                             const int64_t *msg_off, const int32_t *msg_len, const int32_t *msg_key,
+                             const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out);
 
 #ifdef __cplusplus
 }

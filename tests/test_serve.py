@@ -1,0 +1,138 @@
+"""Continuous-batching server: a request gives the same bytes when it is
+served alone, in a mix of overlapping ragged batches, through the batch
+entry points and by the oracle."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(eng, esk, epk, n, nkeys, seed):
+    """n mixed requests over ragged messages (lengths 0, 136, 1 KiB among
+    them); about 40% verifications of batch-made signatures, a quarter of
+    which are corrupted (signature byte or message bit)."""
+    import torch
+    z = eng.sizes
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 1100, n).astype(np.int32)
+    lens[:3] = [0, 136, 1024]
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+    keys = rng.integers(0, nkeys, n).astype(np.int32)
+    dev = eng.device
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sigs = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+    eng.sign_table(esk, t(keys), t(flat), t(offs), t(lens), sigs, eng.sign_workspace(n))
+    torch.cuda.synchronize()
+    batch_sigs = sigs.cpu().numpy()
+    ops = (rng.random(n) < 0.4).astype(np.uint8)
+    ops[:2] = [0, 1]
+    reqs, expect_valid = [], {}
+    for i in range(n):
+        msg = flat[offs[i]:offs[i] + lens[i]].tobytes()
+        if ops[i] == 1:
+            sig = bytearray(batch_sigs[i].tobytes())
+            bad = rng.random() < 0.25
+            if bad:
+                if lens[i] and rng.random() < 0.5:
+                    msg = bytearray(msg)
+                    msg[rng.integers(0, lens[i])] ^= 0x10
+                    msg = bytes(msg)
+                else:
+                    sig[rng.integers(0, z.sig_bytes)] ^= 0x01
+            expect_valid[i] = 1 if bad else 0
+            reqs.append((i, 1, int(keys[i]), msg, bytes(sig)))
+        else:
+            reqs.append((i, 0, int(keys[i]), msg, None))
+    return reqs, batch_sigs, expect_valid
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_serve_alone_equals_mix_equals_batch(engine_factory, version):
+    from cumayo_amd.serve import Server
+    eng = engine_factory(version)
+    nkeys, n = 5, 700
+    esk, epk = eng.keypair_tables(nkeys, rand_index=11)
+    reqs, batch_sigs, expect = _requests(eng, esk, epk, n, nkeys, seed=version)
+    print(f"[serve] MAYO_{version} n={n}", flush=True)
+
+    # in the mix: many small overlapping batches, submitted in bursts
+    with Server(eng, esk, epk, max_batch=48, n_slots=4) as srv:
+        rng = np.random.default_rng(99)
+        i = 0
+        while i < n:
+            k = int(rng.integers(1, 120))
+            assert srv.submit(reqs[i:i + k]) == len(reqs[i:i + k])
+            i += k
+            time.sleep(float(rng.random()) * 0.002)
+        mix = srv.wait(range(n))
+        st = srv.stats()
+    assert st["requests"] == n and st["batches"] >= n // 48
+    assert st["max_in_flight"] >= 2, st
+
+    # alone: one request per batch, nothing else in flight
+    alone_ids = list(range(0, n, 23)) + [0, 1, 2]
+    with Server(eng, esk, epk, max_batch=8, n_slots=2) as srv:
+        alone = {}
+        for j in alone_ids:
+            srv.submit([reqs[j]])
+            alone[j] = srv.wait([j])[j]
+        assert srv.stats()["max_in_flight"] == 1
+
+    for j in range(n):
+        status, _, sig, res = mix[j]
+        if reqs[j][1] == 0:
+            assert status == 0 and res >= 1
+            assert sig.tobytes() == batch_sigs[j].tobytes(), f"sign {j}"
+        else:
+            assert status == expect[j] and res == expect[j], f"verify {j}"
+    for j in alone_ids:
+        a, m = alone[j], mix[j]
+        assert a[0] == m[0] and a[3] == m[3]
+        if reqs[j][1] == 0:
+            assert a[2].tobytes() == m[2].tobytes()
+
+    # oracle on a sample of signatures
+    esk_h = esk.cpu().numpy()
+    for j in [r[0] for r in reqs if r[1] == 0][:12]:
+        osig, _ = O.sign(version, esk_h[reqs[j][2]].tobytes(), reqs[j][3])
+        assert mix[j][2].tobytes() == osig, f"oracle {j}"
+
+
+def test_serve_refusals(engine_factory):
+    from cumayo_amd.serve import STATUS_REFUSED, Server
+    eng = engine_factory(1)
+    esk, epk = eng.keypair_tables(2, rand_index=3)
+    z = eng.sizes
+    with Server(eng, esk, None, max_batch=16, n_slots=2, slot_msg_bytes=4096) as srv:
+        bad = [(0, 0, 7, b"abc", None),                    # key out of range
+               (1, 1, 0, b"abc", bytes(z.sig_bytes)),       # no epk table given
+               (2, 0, 0, bytes(5000), None),                # longer than a slot holds
+               (3, 0, 1, b"fine", None)]
+        assert srv.submit(bad) == 4
+        got = srv.wait(range(4))
+    assert [got[i][0] for i in range(3)] == [STATUS_REFUSED] * 3
+    assert got[3][0] == 0
+    osig, _ = O.sign(1, esk[1].cpu().numpy().tobytes(), b"fine")
+    assert got[3][2].tobytes() == osig
+
+
+def test_serve_poisson_load(engine_factory):
+    """Open-loop Poisson arrivals: everything offered completes, in order of
+    magnitude of the batch latency."""
+    import bench
+    from cumayo_amd.serve import Server
+    eng = engine_factory(1)
+    nkeys, pool = 8, 4096
+    esk, epk = eng.keypair_tables(nkeys, rand_index=1)
+    flat, offs, lens, kidx = bench.make_workload(pool, nkeys, 32, 1024, tag="serve-test")
+    with Server(eng, esk, epk, max_batch=2048, n_slots=3) as srv:
+        rep = srv.run_poisson(200_000.0, 0.5, 0.0, 5, flat, offs, lens, kidx)
+    print(f"[serve] poisson {rep}", flush=True)
+    assert rep["refused"] == 0
+    assert rep["completed"] == rep["submitted"] > 50_000
+    assert 0 < rep["p50_ms"] <= rep["p99_ms"] < 200
