@@ -105,6 +105,9 @@ def main():
     # keys: generated, then expanded straight into dense device tables
     # (pitch = row length), so the resident keys never leave HBM
     d_esk, d_epk = eng.keypair_tables(args.keys, rand_index=0)
+    # the kernel layout of the same keys (entries padded to whole dwords),
+    # built once per key like the expansion itself
+    p_esk, p_epk = eng.prepare_tables(d_esk, d_epk)
     flat, offs, lens, kidx = make_workload(n, args.keys, 32, 1024)
     t = lambda a: torch.from_numpy(np.array(a)).to(dev)
     d_msgs, d_off, d_len, d_kidx = t(flat), t(offs), t(lens), t(kidx)
@@ -116,15 +119,15 @@ def main():
 
     def sign_step(ev=None):
         if ev is None:
-            eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s)
+            eng.sign_prepared(p_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s)
         else:
-            eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s, stages=1)
+            eng.sign_prepared(p_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s, stages=1)
             ev[0].record(stream)
-            eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s, stages=2)
+            eng.sign_prepared(p_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s, stages=2)
             ev[1].record(stream)
 
     def verify_step():
-        eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_v)
+        eng.verify_prepared(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_v)
 
     for _ in range(args.warmup):
         sign_step()
@@ -150,6 +153,40 @@ def main():
     verify_s = (time.perf_counter() - t0) / args.steps
     print(f"[bench] timed: sign {sign_s*1e3:.2f} ms/step verify {verify_s*1e3:.2f} ms/step",
           file=sys.stderr, flush=True)
+
+    # the plain form: one workgroup per item running the whole signing
+    # (or verification) stage in one kernel, the reference's own structure
+    def timed(fn, reps):
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize(dev)
+        return (time.perf_counter() - t0) / reps
+
+    plain_sigs = torch.zeros_like(d_sigs)
+    plain_res = torch.full_like(d_res, 0xEE)
+    plain_sign = lambda: eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, plain_sigs, ws_s, stages=7)
+    plain_verify = lambda: eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, plain_res, ws_v,
+                                            stages=7)
+    plain_sign()
+    plain_verify()
+    plain_sign_s = timed(plain_sign, max(1, args.steps // 2))
+    plain_verify_s = timed(plain_verify, max(1, args.steps // 2))
+    plain_same = bool(torch.equal(plain_sigs, d_sigs)) and bool(torch.equal(plain_res, d_res))
+    # the same wave kernels over the reference-format key tables
+    ref_sigs = torch.zeros_like(d_sigs)
+    ref_res = torch.full_like(d_res, 0xEE)
+    ref_sign = lambda: eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, ref_sigs, ws_s)
+    ref_verify = lambda: eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, ref_res, ws_v)
+    ref_sign()
+    ref_verify()
+    ref_sign_s = timed(ref_sign, max(1, args.steps // 2))
+    ref_verify_s = timed(ref_verify, max(1, args.steps // 2))
+    ref_same = bool(torch.equal(ref_sigs, d_sigs)) and bool(torch.equal(ref_res, d_res))
+    # restore the workspace of the default form for the attempt counts below
+    sign_step()
+    torch.cuda.synchronize(dev)
 
     res = d_res.cpu().numpy()
     all_verify = bool((res == 0).all())
@@ -179,11 +216,21 @@ def main():
         "dtype": "gf16 (u4 nibbles in u32 words)",
         "data": "synthetic (SHAKE-seeded messages, keys from the reference seed rule)",
         "config": {"workload": f"MAYO_{v} sign+verify, {n} signatures, {args.keys} resident keys "
-                               f"(key j mod {args.keys}), message lengths uniform [32,1024] B",
+                               f"(key j mod {args.keys}, prepared key tables), message lengths "
+                               f"uniform [32,1024] B",
                    "global_batch": n, "keys": args.keys, "param_set": f"MAYO_{v}"},
         "verify": {"value": round(verify_per_s, 1), "unit": "verify/s",
                    "ms_per_step": round(verify_s * 1e3, 3), "all_valid": all_verify,
                    "invalid": int(bad.size), "invalid_first": bad[:8].tolist()},
+        "plain_form": {"kernels": "one workgroup per item, whole stage in one kernel",
+                       "sign_per_s": round(n / plain_sign_s, 1),
+                       "verify_per_s": round(n / plain_verify_s, 1),
+                       "speedup_sign": round(plain_sign_s / sign_s, 2),
+                       "speedup_verify": round(plain_verify_s / verify_s, 2),
+                       "identical_outputs": plain_same},
+        "reference_layout_tables": {"sign_per_s": round(n / ref_sign_s, 1),
+                                    "verify_per_s": round(n / ref_verify_s, 1),
+                                    "identical_outputs": ref_same},
         "sign_retries": {"items_with_retry": int((attempts > 1).sum()),
                          "exhausted": int((attempts == 0).sum()),
                          "max_attempts": int(attempts.max())},

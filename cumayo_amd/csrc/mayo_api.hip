@@ -265,14 +265,12 @@ int cumayo_last_error(void) { return g_last_error.exchange(0); }
 
 size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_STRIDE; }
 
-int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
-                             const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
-                             uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
-                             int stages) {
-    if (!n) return 0;
-    if (!esk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !workspace)
-        return (int)hipErrorInvalidValue;
-    TableSign acc{esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, workspace};
+}  // extern "C"
+
+namespace {
+
+template <class KL>
+int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stages) {
     hipError_t e = hipSuccess;
     if (stages & 1) {
         k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
@@ -284,14 +282,103 @@ int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const 
         } else {
             // attempt 0 in two kernels tuned for their register needs, then
             // the rank failures (about 1 in 4,000) through the full loop
-            wv::k_sign_prep<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+            wv::k_sign_prep<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
             wv::k_sign_solve<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
-            wv::k_sign_wave<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1);
+            wv::k_sign_wave<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1);
         }
         e = hipGetLastError();
     }
     note(e);
     return (int)e;
+}
+
+template <class KL>
+int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int stages) {
+    hipError_t e = hipSuccess;
+    if (stages & 1) {
+        k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && (stages & 2)) {
+        if (stages & 4)
+            k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
+        else
+            wv::k_verify_wave<TableVerify, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+        e = hipGetLastError();
+    }
+    note(e);
+    return (int)e;
+}
+
+int prepare_keys(const uint8_t *src, size_t src_stride, size_t n_keys, uint8_t *dst, uint32_t dst_bytes,
+                 uint32_t hdr, wv::PrepRegion r0, wv::PrepRegion r1, wv::PrepRegion r2,
+                 hipStream_t stream) {
+    if (!n_keys) return 0;
+    if (!src || !dst || ((uintptr_t)dst & 15) || n_keys > 65535) return (int)hipErrorInvalidValue;
+    const dim3 grid((dst_bytes / 4 + 255) / 256, (unsigned)n_keys);
+    wv::k_prepare_keys<<<grid, 256, 0, stream>>>(src, src_stride, dst, dst_bytes, dst_bytes, hdr, r0, r1,
+                                                   r2);
+    const hipError_t e = hipGetLastError();
+    note(e);
+    return (int)e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
+                             const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
+                             uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
+                             int stages) {
+    if (!n) return 0;
+    if (!esk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !workspace)
+        return (int)hipErrorInvalidValue;
+    TableSign acc{esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, workspace};
+    return sign_table_impl<wv::RefKeys>(acc, n, stream, stages);
+}
+
+size_t cumayo_prepared_esk_bytes(void) { return wv::PrepKeys::ESK_BYTES; }
+size_t cumayo_prepared_epk_bytes(void) { return wv::PrepKeys::EPK_BYTES; }
+
+int cumayo_prepare_esk(const uint8_t *esk_table, size_t esk_stride, size_t n_keys, uint8_t *prepared,
+                       hipStream_t stream) {
+    using P = wv::PrepKeys;
+    return prepare_keys(esk_table, esk_stride, n_keys, prepared, P::ESK_BYTES, SK + O_BYTES,
+                        {SK + O_BYTES, P::ESK_P1, P1_ENTRIES},
+                        {SK + O_BYTES + P1_BYTES, P::ESK_L, P2_ENTRIES}, {0, 0, 0}, stream);
+}
+
+int cumayo_prepare_epk(const uint8_t *epk_table, size_t epk_stride, size_t n_keys, uint8_t *prepared,
+                       hipStream_t stream) {
+    using P = wv::PrepKeys;
+    return prepare_keys(epk_table, epk_stride, n_keys, prepared, P::EPK_BYTES, 0, {0, P::EPK_P1, P1_ENTRIES},
+                        {P1_BYTES, P::EPK_P2, P2_ENTRIES}, {P1_BYTES + P2_BYTES, P::EPK_P3, P3_ENTRIES},
+                        stream);
+}
+
+int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                         const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                         uint8_t *workspace, hipStream_t stream, int stages) {
+    if (!n) return 0;
+    if (!prepared || ((uintptr_t)prepared & 15) || !key_idx || !msgs || !msg_off || !msg_len || !sigs ||
+        !workspace || (stages & 4))
+        return (int)hipErrorInvalidValue;
+    TableSign acc{prepared, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, workspace};
+    return sign_table_impl<wv::PrepKeys>(acc, n, stream, stages);
+}
+
+int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                           const int64_t *msg_off, const int32_t *msg_len, const uint8_t *sigs,
+                           uint8_t *results, size_t n, uint8_t *workspace, hipStream_t stream,
+                           int stages) {
+    if (!n) return 0;
+    if (!prepared || ((uintptr_t)prepared & 15) || !key_idx || !msgs || !msg_off || !msg_len || !sigs ||
+        !results || !workspace || (stages & 4))
+        return (int)hipErrorInvalidValue;
+    TableVerify acc{prepared, wv::PrepKeys::EPK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, results,
+                    workspace};
+    return verify_table_impl<wv::PrepKeys>(acc, n, stream, stages);
 }
 
 int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
@@ -311,20 +398,7 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
     if (!epk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !results || !workspace)
         return (int)hipErrorInvalidValue;
     TableVerify acc{epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs, results, workspace};
-    hipError_t e = hipSuccess;
-    if (stages & 1) {
-        k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess && (stages & 2)) {
-        if (stages & 4)
-            k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
-        else
-            wv::k_verify_wave<TableVerify><<<(unsigned)n, 64, 0, stream>>>(acc, n);
-        e = hipGetLastError();
-    }
-    note(e);
-    return (int)e;
+    return verify_table_impl<wv::RefKeys>(acc, n, stream, stages);
 }
 
 int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,

@@ -30,10 +30,19 @@ constexpr int NCOL = KO + 1;                   // columns of [A | y']
 constexpr int NCL = (NCOL + 63) / 64;          // columns held per lane
 constexpr int OVW = (PAIRS + 7) / 8 + 1;       // overflow words of an unreduced sum
 constexpr int UW = MW + OVW;                   // words of an unreduced sum
-constexpr int RT = 2;                          // quad-form rows per chunk per lane
-// waves per SIMD the quadratic-form kernels are compiled for: the pair
-// accumulators fit 128 VGPRs up to MAYO_2's sizes, the larger sets use 2
-constexpr int QF_OCC = (M <= 80) ? 4 : 2;
+// quad-form rows per lane per chunk, and the waves per SIMD the quad-form
+// kernels are compiled for (the pair accumulators and QF_RT x 4K bit-plane
+// accumulators must fit the VGPR budget 512 / QF_OCC)
+#ifndef MAYO_QF_RT
+#define MAYO_QF_RT 0
+#endif
+#ifndef MAYO_QF_OCC
+#define MAYO_QF_OCC 0
+#endif
+constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : (K <= 4 ? 2 : K <= 10 ? 2 : 1);
+constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : K <= 11 ? 3 : 2);
+// waves per SIMD of the solve kernel (the bitsliced columns of [A | y'])
+constexpr int SOLVE_OCC = M <= 80 ? 6 : 3;
 // column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
 // go to the last set so that only one set stays active for most pivots
 constexpr int SHIFT = NCOL - 64 * (NCL - 1);
@@ -86,71 +95,249 @@ struct Shared {
     uint8_t hbuf[DIGEST + 2 * SK + 1];  // hash input staging
 };
 
-// ---- quadratic form: Ulds[pair][w] ^= sum_r S_a[r] T_j[r] -------------------
-// P is the n x n form [[P1, P2], [0, P3]] (FULL) or P1 alone (v x v).
-template <bool FULL>
-__device__ __forceinline__ void quad_form(const uint8_t *P1, const uint8_t *P2, const uint8_t *P3,
-                                          const uint64_t *scol, uint32_t *Ulds) {
-    constexpr int NR = FULL ? N : V;
-    constexpr int RPS = (NR + SL - 1) / SL;  // rows per slot
+// ---- key-row layouts ----------------------------------------------------------
+// Reference rows (esk = sk || O || P1 || L, epk = P1 || P2 || P3): entries
+// MB bytes apart at any alignment. Prepared rows: the same blocks with every
+// entry padded with zero bytes to MBP (a multiple of 4) and each block
+// 16-byte aligned, so that a word of an entry is one aligned dword load.
+constexpr int MBP = (MB + 3) & ~3;
+struct RefKeys {
+    static constexpr bool ALIGNED = false;
+    static constexpr int EB = MB;
+    static constexpr uint32_t ESK_P1 = SK + O_BYTES, ESK_L = ESK_P1 + P1_BYTES;
+    static constexpr uint32_t EPK_P1 = 0, EPK_P2 = P1_BYTES, EPK_P3 = P1_BYTES + P2_BYTES;
+};
+struct PrepKeys {
+    static constexpr bool ALIGNED = true;
+    static constexpr int EB = MBP;
+    static constexpr uint32_t ESK_P1 = (SK + O_BYTES + 15) / 16 * 16;
+    static constexpr uint32_t ESK_L = ESK_P1 + P1_ENTRIES * MBP;
+    static constexpr uint32_t ESK_BYTES = (uint32_t)align128(ESK_L + P2_ENTRIES * MBP);
+    static constexpr uint32_t EPK_P1 = 0, EPK_P2 = P1_ENTRIES * MBP;
+    static constexpr uint32_t EPK_P3 = EPK_P2 + P2_ENTRIES * MBP;
+    static constexpr uint32_t EPK_BYTES = (uint32_t)align128(EPK_P3 + P3_ENTRIES * MBP);
+};
+
+// Rewrite reference key rows in the prepared layout. Each destination dword
+// is assembled from the source bytes it maps to: the header bytes [0, hdr)
+// are copied, region r maps entry e (MB bytes at src + e MB) to dst + e MBP
+// with zero padding, everything else is zero. Grid: (dwords per row / 256
+// rounded up) x keys.
+struct PrepRegion {
+    uint32_t src, dst, count;
+};
+__global__ void __launch_bounds__(256) k_prepare_keys(const uint8_t *__restrict__ src, size_t src_stride,
+                                                      uint8_t *__restrict__ dst, size_t dst_stride,
+                                                      uint32_t dst_bytes, uint32_t hdr, PrepRegion r0,
+                                                      PrepRegion r1, PrepRegion r2) {
+    const uint32_t o = 4 * (blockIdx.x * 256 + threadIdx.x);
+    if (o >= dst_bytes) return;
+    const uint8_t *s = src + (size_t)blockIdx.y * src_stride;
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t ob = o + i;
+        uint32_t byte = 0;
+        if (ob < hdr) {
+            byte = s[ob];
+        } else {
+            const PrepRegion rg[3] = {r0, r1, r2};
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (ob >= rg[r].dst && ob < rg[r].dst + rg[r].count * (uint32_t)MBP) {
+                    const uint32_t e = (ob - rg[r].dst) / MBP, q = (ob - rg[r].dst) % MBP;
+                    if (q < (uint32_t)MB) byte = s[rg[r].src + e * MB + q];
+                }
+            }
+        }
+        v |= byte << (8 * i);
+    }
+    *(uint32_t *)(dst + (size_t)blockIdx.y * dst_stride + o) = v;
+}
+
+// Word w of a key entry (byte offset off from base, nv valid bytes), in two
+// steps so that the loads of the next column are in flight while the
+// current one is used: fetch issues the loads, word() assembles the value.
+template <class KL>
+struct EntryLoad {
+    uint32_t lo, hi, sh;
+    __device__ __forceinline__ void fetch(const uint8_t *base, uint32_t off, int nv) {
+        if constexpr (KL::ALIGNED) {
+            lo = *(const uint32_t *)(base + off);  // padding bytes are zero
+        } else {
+            const uintptr_t a = (uintptr_t)(base + off);
+            sh = (uint32_t)(a & 3);
+            const uint32_t *q = (const uint32_t *)(a - sh);
+            lo = q[0];
+            hi = (sh + (uint32_t)nv > 4) ? q[1] : 0u;  // only the dwords holding the entry's bytes
+        }
+    }
+    __device__ __forceinline__ uint32_t word(int nv) const {
+        if constexpr (KL::ALIGNED) {
+            return lo;
+        } else {
+            const uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, sh);
+            return nv < 4 ? v & ((1u << (8 * nv)) - 1u) : v;
+        }
+    }
+};
+
+// ---- quadratic form -------------------------------------------------------------
+// T_j[row] = sum_c P[row][c] S_j[c] for every row a lane owns, with c uniform
+// across the wave. Writing S_j[c] = sum_b s_jb(c) a^b (bits s_jb), T_j =
+// sum_b a^b T_jb with T_jb = XOR of the entries whose scalar has bit b set:
+// one AND-XOR per entry word and scalar bit (SGPR masks), and the powers of
+// a are applied once per row at the end (Horner), not per entry.
+//
+// Rows: with NLR > 0 (signing) the first NLR rows are the columns x of L,
+// entries L[c][x] over every c, so T_j of row x is M_j[x] (sign.cu:256-330);
+// then the rows r of P1 (entries c >= r; sign.cu:395-437). With FULL
+// (verification) the rows are those of [[P1, P2], [0, P3]] over c < n
+// (verify.cu:72-182). Each P row then adds S_a[r] T_j[r] into the pair
+// accumulators U[pair(a, j)] (ordered (a, j) both ways), XOR-reduced into
+// Ulds at the end; each L row stores M_j[x] into Mv.
+//
+// Lanes = (word w, row slot); a chunk gives each slot QF_RT rows, rows in
+// ascending start column so that every row of a chunk starts within a few
+// columns of the first (those columns are masked per lane).
+template <class KL, int NLR, bool FULL>
+__device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *scol, uint32_t *Ulds,
+                                          uint32_t *Mv) {
+    constexpr int NP = FULL ? N : V;          // P rows and columns
+    constexpr int NVR = NLR + NP;             // rows of the schedule
+    constexpr int RPC = QF_RT * SL;           // rows per chunk
+    constexpr int NCH = (NVR + RPC - 1) / RPC;
+    constexpr int EB = KL::EB;
+    constexpr uint32_t P1o = FULL ? KL::EPK_P1 : KL::ESK_P1;
     const int lane = threadIdx.x;
     const int w = lane % MW, slot = lane / MW;
     const bool active = slot < SL;
     const int nv = mvec::word_valid(w);
+    auto start_of = [](int vr) { return vr < NLR ? 0 : vr - NLR; };
     uint32_t U[PAIRS];
 #pragma unroll
     for (int p = 0; p < PAIRS; ++p) U[p] = 0;
-    for (int t0 = 0; t0 < RPS; t0 += RT) {
-        uint32_t T[RT][K];
-#pragma unroll
-        for (int tt = 0; tt < RT; ++tt)
-#pragma unroll
-            for (int j = 0; j < K; ++j) T[tt][j] = 0;
-        const int cstart = t0 * SL;
 #pragma unroll 1
-        for (int c = cstart; c < NR; ++c) {
-            const uint64_t sc = uni64(scol[c]);
-            // multiples of this lane's entries (r, c) of the chunk; a row not
-            // yet reached (r > c) or past the end contributes zero
-            uint32_t x0[RT], x1[RT], x2[RT], x3[RT];
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int v0 = ch * RPC;
+        const int vlast = (v0 + RPC < NVR ? v0 + RPC : NVR) - 1;
+        const int c_lo = start_of(v0), c_free = start_of(vlast);
+        int st[QF_RT];
+        uint32_t off[QF_RT], stride[QF_RT];
+        uint32_t Tb[QF_RT][K][4];
 #pragma unroll
-            for (int tt = 0; tt < RT; ++tt) {
-                const int r = slot + SL * (t0 + tt);
-                uint32_t xv = 0;
-                if (active && r <= c && r < NR) {
-                    const uint8_t *e;
-                    if (!FULL || (r < V && c < V)) e = P1 + (size_t)tri_entry(r, c, V) * MB;
-                    else if (r < V) e = P2 + (size_t)(r * O + (c - V)) * MB;
-                    else e = P3 + (size_t)tri_entry(r - V, c - V, O) * MB;
-                    xv = mvec::load_word(e + 4 * w, nv);
+        for (int tt = 0; tt < QF_RT; ++tt) {
+            const int vr = v0 + tt * SL + slot;
+            const bool valid = active && vr < NVR;
+            st[tt] = valid ? start_of(vr) : NP;
+            off[tt] = 0;
+            stride[tt] = 0;
+            if (valid) {
+                if (vr < NLR) {
+                    off[tt] = KL::ESK_L + (uint32_t)((c_lo * O + vr) * EB);
+                    stride[tt] = O * EB;
+                } else {
+                    const int r = vr - NLR;
+                    if (r < V) {
+                        off[tt] = P1o + (uint32_t)((tri_entry(r, r, V) - (r - c_lo)) * EB);
+                        stride[tt] = EB;
+                    }
                 }
-                x0[tt] = xv; x1[tt] = gf16::mul_a(xv); x2[tt] = gf16::mul_a(x1[tt]); x3[tt] = gf16::mul_a(x2[tt]);
             }
+            off[tt] += 4 * w;
 #pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const uint32_t sj = (uint32_t)(sc >> (4 * j));
-                const uint32_t m0 = gf16::bitmask(sj, 0), m1 = gf16::bitmask(sj, 1);
-                const uint32_t m2 = gf16::bitmask(sj, 2), m3 = gf16::bitmask(sj, 3);
+            for (int j = 0; j < K; ++j)
 #pragma unroll
-                for (int tt = 0; tt < RT; ++tt)
-                    T[tt][j] = gf16::mac(T[tt][j], x0[tt], x1[tt], x2[tt], x3[tt], m0, m1, m2, m3);
-            }
+                for (int b = 0; b < 4; ++b) Tb[tt][j][b] = 0;
         }
+        // one column: every row of the chunk takes its entry at c (fetched
+        // during the previous column); the next column's loads are issued
+        // before the accumulation when the segment has one
+        EntryLoad<KL> ld[QF_RT];
+        auto fetch = [&]() {
 #pragma unroll
-        for (int tt = 0; tt < RT; ++tt) {
-            const int r = slot + SL * (t0 + tt);
-            if (active && r < NR) {
-                const uint64_t sr = scol[r];
+            for (int tt = 0; tt < QF_RT; ++tt) ld[tt].fetch(key, off[tt], nv);
+        };
+        auto column = [&](int c, bool masked, bool more) {
+            const uint64_t sc = uni64(scol[c]);
+            uint32_t x[QF_RT];
+#pragma unroll
+            for (int tt = 0; tt < QF_RT; ++tt) {
+                x[tt] = ld[tt].word(nv);
+                if (masked) x[tt] = c >= st[tt] ? x[tt] : 0u;
+                off[tt] += stride[tt];
+            }
+            if (more) fetch();
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t m = (uint32_t)(-(int32_t)((sc >> (4 * j + b)) & 1u));
+#pragma unroll
+                    for (int tt = 0; tt < QF_RT; ++tt) Tb[tt][j][b] = gf16::axm(Tb[tt][j][b], x[tt], m);
+                }
+        };
+        // columns < V (P1 and L entries)
+        if (c_lo < V) {
+            const int cm = c_free < V ? c_free : V;
+            fetch();
+#pragma unroll 1
+            for (int c = c_lo; c < cm; ++c) column(c, true, c + 1 < V);
+#pragma unroll 1
+            for (int c = cm; c < V; ++c) column(c, false, c + 1 < V);
+        }
+        if constexpr (FULL) {
+            // columns >= V: P2 for rows < V, P3 for rows >= V
+            const int cb = c_lo > V ? c_lo : V;
+#pragma unroll
+            for (int tt = 0; tt < QF_RT; ++tt) {
+                const int r = v0 + tt * SL + slot;
+                if (active && r < NP) {
+                    if (r < V) {
+                        off[tt] = KL::EPK_P2 + (uint32_t)((r * O + (cb - V)) * EB) + 4 * w;
+                    } else {
+                        const int r3 = r - V;
+                        off[tt] = KL::EPK_P3 +
+                                  (uint32_t)((tri_entry(r3, r3, O) - (r3 - (cb - V))) * EB) + 4 * w;
+                    }
+                    stride[tt] = EB;
+                }
+            }
+            const int cm = c_free > cb ? c_free : cb;
+            fetch();
+#pragma unroll 1
+            for (int c = cb; c < cm; ++c) column(c, true, c + 1 < N);
+#pragma unroll 1
+            for (int c = cm; c < N; ++c) column(c, false, c + 1 < N);
+        }
+        // rows done: T_j = sum_b a^b T_jb
+#pragma unroll
+        for (int tt = 0; tt < QF_RT; ++tt) {
+            const int vr = v0 + tt * SL + slot;
+            uint32_t T[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                T[j] = gf16::mul_a(gf16::mul_a(gf16::mul_a(Tb[tt][j][3]) ^ Tb[tt][j][2]) ^ Tb[tt][j][1]) ^
+                       Tb[tt][j][0];
+            if (!active || vr >= NVR) continue;
+            if (vr < NLR) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) Mv[(j * O + vr) * MW + w] = T[j];
+            } else {
+                const uint64_t sr = scol[vr - NLR];
+                uint32_t am[K][4];
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) am[a][b] = gf16::bitmask((uint32_t)(sr >> (4 * a)), b);
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    gf16::Multiples mt(T[tt][j]);
+                    const gf16::Multiples mt(T[j]);
 #pragma unroll
                     for (int a = 0; a < K; ++a) {
                         const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
-                        const uint32_t sa = (uint32_t)(sr >> (4 * a));
-                        U[p] = gf16::mac(U[p], mt.x0, mt.x1, mt.x2, mt.x3, gf16::bitmask(sa, 0),
-                                         gf16::bitmask(sa, 1), gf16::bitmask(sa, 2),
-                                         gf16::bitmask(sa, 3));
+                        U[p] = gf16::mac(U[p], mt.x0, mt.x1, mt.x2, mt.x3, am[a][0], am[a][1], am[a][2],
+                                         am[a][3]);
                     }
                 }
             }
@@ -444,36 +631,6 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
     return ok;
 }
 
-// M_i = V_i L (sign.cu:256-330): lanes = (column x, word w) items, r uniform.
-__device__ __forceinline__ void wave_M(Shared &S, const uint8_t *L) {
-    const int lane = threadIdx.x;
-    for (int item = lane; item < ((O * MW + 63) / 64) * 64; item += 64) {
-        const bool act = item < O * MW;
-        const int x = act ? item / MW : 0, w = act ? item % MW : 0;
-        const int nv = mvec::word_valid(w);
-        uint32_t a[K];
-#pragma unroll
-        for (int i = 0; i < K; ++i) a[i] = 0;
-        for (int r = 0; r < V; ++r) {
-            const uint64_t sc = uni64(S.scol[r]);
-            if (act) {
-                gf16::Multiples ml(mvec::load_word(L + (size_t)(r * O + x) * MB + 4 * w, nv));
-#pragma unroll
-                for (int i = 0; i < K; ++i) {
-                    const uint32_t si = (uint32_t)(sc >> (4 * i));
-                    a[i] = gf16::mac(a[i], ml.x0, ml.x1, ml.x2, ml.x3, gf16::bitmask(si, 0),
-                                     gf16::bitmask(si, 1), gf16::bitmask(si, 2),
-                                     gf16::bitmask(si, 3));
-                }
-            }
-        }
-        if (act) {
-#pragma unroll
-            for (int i = 0; i < K; ++i) S.Mv[(i * O + x) * MW + w] = a[i];
-        }
-    }
-}
-
 __device__ __forceinline__ void wave_scol(Shared &S) {
     for (int c = threadIdx.x; c < V; c += 64) {
         uint64_t v = 0;
@@ -509,14 +666,12 @@ __device__ __forceinline__ void wave_write_sig(const Shared &S, const SignItem &
 
 // The whole signing loop in one wave (sign.cu:244-478). With retry_only set
 // it handles only items the solve kernel left pending, from attempt 1 on.
-template <class Acc>
-__global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n, int retry_only) {
+template <class Acc, class KL = RefKeys>
+__global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int retry_only) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
     if (retry_only && it.scratch[SCR_ATTEMPTS] != ATTEMPT_PENDING) return;
-    const uint8_t *P1 = it.esk + SK + O_BYTES;
-    const uint8_t *L = P1 + P1_BYTES;
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
     int ctr = retry_only ? 1 : 0;
@@ -540,8 +695,7 @@ __global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n, int retr
         wave_scol(S);
         for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
         __syncthreads();
-        wave_M(S, L);
-        quad_form<false>(P1, nullptr, nullptr, S.scol, S.U);
+        quad_form<KL, O, false>(it.esk, S.scol, S.U, S.Mv);
         __syncthreads();
         wave_y(S, S.t);
         uint32_t cols[NCL][4][NG];
@@ -557,21 +711,18 @@ __global__ void __launch_bounds__(64, 3) k_sign_wave(Acc acc, size_t n, int retr
 
 // Batch form, attempt 0, part 1: M, U and y, handed over through the
 // workspace. Register-heavy (the pair accumulators of the quadratic form).
-template <class Acc>
+template <class Acc, class KL = RefKeys>
 __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
-    const uint8_t *P1 = it.esk + SK + O_BYTES;
-    const uint8_t *L = P1 + P1_BYTES;
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     __syncthreads();
     wave_scol(S);
     for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
     __syncthreads();
-    wave_M(S, L);
-    quad_form<false>(P1, nullptr, nullptr, S.scol, S.U);
+    quad_form<KL, O, false>(it.esk, S.scol, S.U, S.Mv);
     __syncthreads();
     wave_y(S, S.t);
     uint32_t *mv = (uint32_t *)(it.scratch + SCR_MV);
@@ -583,7 +734,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
 // Batch form, attempt 0, part 2: columns of [A | y - A r], Gauss-Jordan,
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
 template <class Acc>
-__global__ void __launch_bounds__(64, 8) k_sign_solve(Acc acc, size_t n) {
+__global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
@@ -606,7 +757,7 @@ __global__ void __launch_bounds__(64, 8) k_sign_solve(Acc acc, size_t n) {
 }
 
 // Verification, one wave per item (verify.cu:9-211).
-template <class Acc>
+template <class Acc, class KL = RefKeys>
 __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
     __shared__ uint64_t scol[N];
     __shared__ uint32_t U[PAIRS * MW];
@@ -615,7 +766,6 @@ __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
     __shared__ uint8_t hbuf[DIGEST + SK];
     const int lane = threadIdx.x;
     VerifyItem it = acc(blockIdx.x);
-    const uint8_t *P1 = it.epk, *P2 = P1 + P1_BYTES, *P3 = P2 + P2_BYTES;
     for (int c = lane; c < N; c += 64) {
         uint64_t v = 0;
         for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(it.sig, j * N + c) << (4 * j);
@@ -632,7 +782,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
         keccak::wave_shake256(tb, MB, hbuf, DIGEST + SK);
     }
     __syncthreads();
-    quad_form<true>(P1, P2, P3, scol, U);
+    quad_form<KL, 0, true>(it.epk, scol, U, nullptr);
     __syncthreads();
     if (lane < UW) {
         uint32_t a = 0;

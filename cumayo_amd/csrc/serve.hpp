@@ -1,6 +1,8 @@
 // Continuous-batching request server over the batched sign / verify kernels
 // (host C++; included by the C ABI translation unit).
 //
+// The server keeps its own prepared copies of the key tables it is given
+// (the padded kernel layout), resident for its lifetime.
 // Requests queue on the host. A worker thread owns n_slots batch slots, each
 // with its own HIP stream, pinned staging and device buffers. Whenever a slot
 // is free and requests are waiting it admits as many as fit (max_batch
@@ -29,6 +31,14 @@
 
 namespace serve {
 
+// Attempt counters of a signing batch, gathered from the per-item workspace
+// slots into a dense byte array (one contiguous copy back instead of one
+// strided row per request).
+__global__ void k_gather_attempts(const uint8_t *__restrict__ ws, uint8_t *__restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ws[i * mayo::SIGN_SCRATCH_STRIDE + mayo::SCR_ATTEMPTS];
+}
+
 inline int64_t now_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(
                std::chrono::steady_clock::now().time_since_epoch())
@@ -42,7 +52,8 @@ struct Slot {
     uint8_t *h_msgs = nullptr, *h_sig = nullptr, *h_res = nullptr, *h_att = nullptr;
     int64_t *h_off = nullptr;
     int32_t *h_len = nullptr, *h_key = nullptr;
-    uint8_t *d_msgs = nullptr, *d_sig = nullptr, *d_res = nullptr, *d_wss = nullptr, *d_wsv = nullptr;
+    uint8_t *d_msgs = nullptr, *d_sig = nullptr, *d_res = nullptr, *d_att = nullptr, *d_wss = nullptr,
+            *d_wsv = nullptr;
     int64_t *d_off = nullptr;
     int32_t *d_len = nullptr, *d_key = nullptr;
     size_t n_sign = 0, n_verify = 0;
@@ -56,6 +67,18 @@ class Server {
         if (cfg_.n_slots <= 0) cfg_.n_slots = 3;
         if (!cfg_.slot_msg_bytes) cfg_.slot_msg_bytes = (size_t)cfg_.max_batch * 1024;
         ok(hipSetDevice(cfg_.device));
+        // resident keys in the kernel layout, built once on the device
+        if (cfg_.esk_table && cfg_.n_keys > 0) {
+            dev_alloc(&pesk_, (size_t)cfg_.n_keys * cumayo_prepared_esk_bytes());
+            if (pesk_) ok((hipError_t)cumayo_prepare_esk(cfg_.esk_table, cfg_.esk_stride, (size_t)cfg_.n_keys,
+                                                        pesk_, nullptr));
+        }
+        if (cfg_.epk_table && cfg_.n_keys > 0) {
+            dev_alloc(&pepk_, (size_t)cfg_.n_keys * cumayo_prepared_epk_bytes());
+            if (pepk_) ok((hipError_t)cumayo_prepare_epk(cfg_.epk_table, cfg_.epk_stride, (size_t)cfg_.n_keys,
+                                                        pepk_, nullptr));
+        }
+        ok(hipDeviceSynchronize());
         const size_t mb = (size_t)cfg_.max_batch;
         slots_.resize((size_t)cfg_.n_slots);
         for (auto &s : slots_) {
@@ -74,6 +97,7 @@ class Server {
             dev_alloc((uint8_t **)&s.d_key, mb * sizeof(int32_t));
             dev_alloc(&s.d_sig, mb * mayo::SIG_BYTES);
             dev_alloc(&s.d_res, mb);
+            dev_alloc(&s.d_att, mb);
             dev_alloc(&s.d_wss, mb * (size_t)mayo::SIGN_SCRATCH_STRIDE);
             dev_alloc(&s.d_wsv, mb * (size_t)mayo::VERIFY_SCRATCH_STRIDE);
             s.reqs.reserve(mb);
@@ -95,11 +119,13 @@ class Server {
             (void)hipHostFree(s.h_key); (void)hipHostFree(s.h_sig); (void)hipHostFree(s.h_res);
             (void)hipHostFree(s.h_att);
             (void)hipFree(s.d_msgs); (void)hipFree(s.d_off); (void)hipFree(s.d_len);
-            (void)hipFree(s.d_key); (void)hipFree(s.d_sig); (void)hipFree(s.d_res);
+            (void)hipFree(s.d_key); (void)hipFree(s.d_sig); (void)hipFree(s.d_res); (void)hipFree(s.d_att);
             (void)hipFree(s.d_wss); (void)hipFree(s.d_wsv);
             if (s.done) (void)hipEventDestroy(s.done);
             if (s.stream) (void)hipStreamDestroy(s.stream);
         }
+        (void)hipFree(pesk_);
+        (void)hipFree(pepk_);
     }
 
     size_t submit(const cumayo_request *reqs, size_t n) {
@@ -231,18 +257,18 @@ class Server {
             ok(hipMemcpyAsync(s.d_sig + s.n_sign * mayo::SIG_BYTES, s.h_sig + s.n_sign * mayo::SIG_BYTES,
                               s.n_verify * mayo::SIG_BYTES, hipMemcpyHostToDevice, st));
         if (s.n_sign) {
-            int e = cumayo_sign_table_stages(cfg_.esk_table, cfg_.esk_stride, s.d_key, s.d_msgs, s.d_off,
-                                             s.d_len, s.d_sig, s.n_sign, s.d_wss, st, 3);
+            int e = cumayo_sign_prepared(pesk_, s.d_key, s.d_msgs, s.d_off, s.d_len, s.d_sig, s.n_sign,
+                                         s.d_wss, st, 3);
             if (e) err_.store(e);
             ok(hipMemcpyAsync(s.h_sig, s.d_sig, s.n_sign * mayo::SIG_BYTES, hipMemcpyDeviceToHost, st));
-            ok(hipMemcpy2DAsync(s.h_att, 1, s.d_wss + mayo::SCR_ATTEMPTS, mayo::SIGN_SCRATCH_STRIDE, 1,
-                                s.n_sign, hipMemcpyDeviceToHost, st));
+            k_gather_attempts<<<(unsigned)((s.n_sign + 255) / 256), 256, 0, st>>>(s.d_wss, s.d_att, s.n_sign);
+            ok(hipGetLastError());
+            ok(hipMemcpyAsync(s.h_att, s.d_att, s.n_sign, hipMemcpyDeviceToHost, st));
         }
         if (s.n_verify) {
-            int e = cumayo_verify_table_stages(cfg_.epk_table, cfg_.epk_stride, s.d_key + s.n_sign, s.d_msgs,
-                                               s.d_off + s.n_sign, s.d_len + s.n_sign,
-                                               s.d_sig + s.n_sign * mayo::SIG_BYTES, s.d_res, s.n_verify,
-                                               s.d_wsv, st, 3);
+            int e = cumayo_verify_prepared(pepk_, s.d_key + s.n_sign, s.d_msgs, s.d_off + s.n_sign,
+                                           s.d_len + s.n_sign, s.d_sig + s.n_sign * mayo::SIG_BYTES, s.d_res,
+                                           s.n_verify, s.d_wsv, st, 3);
             if (e) err_.store(e);
             ok(hipMemcpyAsync(s.h_res, s.d_res, s.n_verify, hipMemcpyDeviceToHost, st));
         }
@@ -282,6 +308,7 @@ class Server {
     }
 
     cumayo_serve_config cfg_;
+    uint8_t *pesk_ = nullptr, *pepk_ = nullptr;  // prepared key tables
     std::vector<Slot> slots_;
     std::mutex mu_;
     std::condition_variable cv_;

@@ -101,6 +101,12 @@ EXPORTS = {
     "cumayo_verify_table": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
     "cumayo_sign_table_stages": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
     "cumayo_verify_table_stages": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
+    "cumayo_prepared_esk_bytes": [],
+    "cumayo_prepared_epk_bytes": [],
+    "cumayo_prepare_esk": [_VP, _SZ, _SZ, _VP, _VP],
+    "cumayo_prepare_epk": [_VP, _SZ, _SZ, _VP, _VP],
+    "cumayo_sign_prepared": [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
+    "cumayo_verify_prepared": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
 }
 _RESTYPES = {
     "cumayo_last_error": ctypes.c_int,
@@ -110,6 +116,12 @@ _RESTYPES = {
     "cumayo_verify_table": ctypes.c_int,
     "cumayo_sign_table_stages": ctypes.c_int,
     "cumayo_verify_table_stages": ctypes.c_int,
+    "cumayo_prepared_esk_bytes": _SZ,
+    "cumayo_prepared_epk_bytes": _SZ,
+    "cumayo_prepare_esk": ctypes.c_int,
+    "cumayo_prepare_epk": ctypes.c_int,
+    "cumayo_sign_prepared": ctypes.c_int,
+    "cumayo_verify_prepared": ctypes.c_int,
 }
 
 
@@ -389,3 +401,42 @@ class Engine:
             self.stream_handle(stream), stages)
         if err:
             raise RuntimeError(f"cumayo_verify_table failed: HIP error {err}")
+
+    # -- prepared key tables ------------------------------------------------
+    def prepare_tables(self, esk_table=None, epk_table=None, stream=None):
+        """Prepared (padded, aligned) copies of reference-format key tables
+        (device tensors, one key per row); None where no table is given."""
+        torch = self.torch
+        st = self.stream_handle(stream)
+        out = []
+        for tab, nbytes, fn in ((esk_table, self.lib.cumayo_prepared_esk_bytes(), self.lib.cumayo_prepare_esk),
+                                (epk_table, self.lib.cumayo_prepared_epk_bytes(), self.lib.cumayo_prepare_epk)):
+            if tab is None:
+                out.append(None)
+                continue
+            nk = tab.shape[0]
+            prep = torch.empty((nk, int(nbytes)), dtype=torch.uint8, device=self.device)
+            err = fn(tab.data_ptr(), tab.stride(0), nk, prep.data_ptr(), st)
+            if err:
+                raise RuntimeError(f"cumayo_prepare_* failed: HIP error {err}")
+            out.append(prep)
+        return tuple(out)
+
+    def sign_prepared(self, prep_esk, key_idx, msgs, msg_off, msg_len, sigs, workspace, stream=None,
+                      stages=3):
+        n = int(key_idx.numel())
+        err = self.lib.cumayo_sign_prepared(
+            prep_esk.data_ptr(), key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
+            msg_len.data_ptr(), sigs.data_ptr(), n, workspace.data_ptr(), self.stream_handle(stream), stages)
+        if err:
+            raise RuntimeError(f"cumayo_sign_prepared failed: HIP error {err}")
+
+    def verify_prepared(self, prep_epk, key_idx, msgs, msg_off, msg_len, sigs, results, workspace,
+                        stream=None, stages=3):
+        n = int(key_idx.numel())
+        err = self.lib.cumayo_verify_prepared(
+            prep_epk.data_ptr(), key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
+            msg_len.data_ptr(), sigs.data_ptr(), results.data_ptr(), n, workspace.data_ptr(),
+            self.stream_handle(stream), stages)
+        if err:
+            raise RuntimeError(f"cumayo_verify_prepared failed: HIP error {err}")

@@ -149,12 +149,37 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
                                const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
                                hipStream_t stream, int stages);
 
+/* Prepared key tables: the reference's esk blocks (sk || O || P1 || L) and
+ * epk blocks (P1 || P2 || P3) with every m-vector entry padded with zero
+ * bytes to a multiple of 4 bytes and every block 16-byte aligned, rows
+ * cumayo_prepared_esk_bytes() / cumayo_prepared_epk_bytes() apart (multiples
+ * of 128). Built once per key on the device from a reference-format table;
+ * the kernels then read each entry word with one aligned load. `prepared`
+ * must be 16-byte aligned; n_keys <= 65535 per call. */
+size_t cumayo_prepared_esk_bytes(void);
+size_t cumayo_prepared_epk_bytes(void);
+int cumayo_prepare_esk(const uint8_t *esk_table, size_t esk_stride, size_t n_keys, uint8_t *prepared,
+                       hipStream_t stream);
+int cumayo_prepare_epk(const uint8_t *epk_table, size_t epk_stride, size_t n_keys, uint8_t *prepared,
+                       hipStream_t stream);
+/* cumayo_sign_table_stages / cumayo_verify_table_stages over prepared
+ * tables (same results; stage bit 4 is not available here). */
+int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                         const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                         uint8_t *workspace, hipStream_t stream, int stages);
+int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                           const int64_t *msg_off, const int32_t *msg_len, const uint8_t *sigs,
+                           uint8_t *results, size_t n, uint8_t *workspace, hipStream_t stream,
+                           int stages);
+
 /* ---------------- Part 3: continuous-batching server ---------------- */
 
 /* Requests queue on the host; a worker thread admits them into free batch
  * slots (one HIP stream, pinned staging and device buffers each) as earlier
  * batches finish, so batches launched while others run overlap on the GPU.
- * Device memory is bounded by n_slots x (slot_msg_bytes + per-request
+ * Device memory is bounded by the prepared copies of the key tables the
+ * server keeps (n_keys rows of cumayo_prepared_esk_bytes() and
+ * cumayo_prepared_epk_bytes()) plus n_slots x (slot_msg_bytes + per-request
  * buffers for max_batch requests); a request waits until a slot with room
  * frees up. Results are identical to the batch entry points. No counterpart
  * in the reference. */

@@ -139,11 +139,19 @@ def test_table_form_large_batch_includes_retries(engine_factory):
     print(f"[parity] items with retries: {int((att > 1).sum())}", flush=True)
     assert (att >= 1).all()
     assert res.cpu().numpy().tolist() == [0] * n
+    # the same batch over prepared key tables (the retry pass included)
+    p_esk, _ = eng.prepare_tables(d_esk, None)
+    psigs = torch.zeros_like(sigs)
+    pws = eng.sign_workspace(n)
+    eng.sign_prepared(p_esk, t(kidx), t(flat), t(offs), t(lens), psigs, pws)
+    torch.cuda.synchronize()
+    assert np.array_equal(eng.sign_attempts(pws, n), att)
     esk = d_esk.cpu().numpy()
     osigs = O.sign_batch(v, esk, kidx, flat, offs, lens, threads=min(os.cpu_count() or 1, 16))
     g = sigs.cpu().numpy()
     bad = np.where((g != osigs).any(axis=1))[0]
     assert bad.size == 0, f"mismatching items {bad[:10].tolist()} attempts {att[bad[:10]].tolist()}"
+    assert np.array_equal(psigs.cpu().numpy(), g)
 
 
 def test_device_key_tables_match_oracle(engine_factory):
@@ -156,3 +164,81 @@ def test_device_key_tables_match_oracle(engine_factory):
             opk, osk = O.keypair(v, 5, i)
             assert esk[i].tobytes() == O.expandsk(v, osk)
             assert epk[i].tobytes() == O.expandpk(v, opk)
+
+
+def _prepared_layout(z, table, kind):
+    """numpy restatement of the prepared layout: entries padded to a
+    multiple of 4 bytes, blocks 16-byte aligned, rows a multiple of 128."""
+    mb = z.m // 2
+    mbp = (mb + 3) // 4 * 4
+    v, o = z.n - z.o, z.o
+    tri_v, tri_o = v * (v + 1) // 2, o * (o + 1) // 2
+    if kind == "esk":
+        hdr = z.sk_bytes + z.o_bytes
+        blocks = [(hdr, tri_v), (hdr + tri_v * mb, v * o)]
+        dst0 = (hdr + 15) // 16 * 16
+    else:
+        hdr = 0
+        blocks = [(0, tri_v), (tri_v * mb, v * o), (tri_v * mb + v * o * mb, tri_o)]
+        dst0 = 0
+    size = dst0 + sum(c for _, c in blocks) * mbp
+    size = (size + 127) // 128 * 128
+    out = np.zeros((table.shape[0], size), np.uint8)
+    out[:, :hdr] = table[:, :hdr]
+    d = dst0
+    for s0, cnt in blocks:
+        src = table[:, s0:s0 + cnt * mb].reshape(-1, cnt, mb)
+        dst = out[:, d:d + cnt * mbp].reshape(-1, cnt, mbp)
+        dst[:, :, :mb] = src
+        d += cnt * mbp
+    return out
+
+
+@pytest.mark.parametrize("v", VERSIONS, ids=lambda v: f"MAYO_{v}")
+def test_prepared_tables_match_reference_layout(engine_factory, v):
+    """Prepared key tables: the layout matches its numpy restatement, and
+    signing / verification over them give the reference-layout results and
+    the oracle's signatures, for ragged messages (empty, one SHAKE block)."""
+    import torch
+    eng = engine_factory(v)
+    z = eng.sizes
+    nkeys, n = 5, 300
+    esk, epk = eng.keypair_tables(nkeys, rand_index=21)
+    pesk, pepk = eng.prepare_tables(esk, epk)
+    torch.cuda.synchronize()
+    assert np.array_equal(pesk.cpu().numpy(), _prepared_layout(z, esk.cpu().numpy(), "esk"))
+    assert np.array_equal(pepk.cpu().numpy(), _prepared_layout(z, epk.cpu().numpy(), "epk"))
+    rng = np.random.default_rng(100 + v)
+    lens = rng.integers(0, 700, n).astype(np.int32)
+    lens[:2] = [0, 136]
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+    kidx = rng.integers(0, nkeys, n).astype(np.int32)
+    dev = eng.device
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d_k, d_m, d_o, d_l = t(kidx), t(flat), t(offs), t(lens)
+    s_ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+    s_prep = torch.zeros_like(s_ref)
+    print(f"[parity] prepared MAYO_{v} n={n}", flush=True)
+    eng.sign_table(esk, d_k, d_m, d_o, d_l, s_ref, eng.sign_workspace(n))
+    eng.sign_prepared(pesk, d_k, d_m, d_o, d_l, s_prep, eng.sign_workspace(n))
+    r_ref = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    r_prep = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    eng.verify_table(epk, d_k, d_m, d_o, d_l, s_prep, r_ref, eng.verify_workspace(n))
+    eng.verify_prepared(pepk, d_k, d_m, d_o, d_l, s_prep, r_prep, eng.verify_workspace(n))
+    bad = s_prep.clone()
+    bad[::3, 7] ^= 0x20
+    r_bad = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    eng.verify_prepared(pepk, d_k, d_m, d_o, d_l, bad, r_bad, eng.verify_workspace(n))
+    torch.cuda.synchronize()
+    assert torch.equal(s_ref, s_prep)
+    assert r_ref.cpu().numpy().tolist() == [0] * n
+    assert r_prep.cpu().numpy().tolist() == [0] * n
+    expect_bad = np.zeros(n, np.uint8)
+    expect_bad[::3] = 1
+    assert np.array_equal(r_bad.cpu().numpy(), expect_bad)
+    esk_h = esk.cpu().numpy()
+    sp = s_prep.cpu().numpy()
+    for i in list(range(0, n, 37)) + [0, 1]:
+        osig, _ = O.sign(v, esk_h[kidx[i]].tobytes(), flat[offs[i]:offs[i] + lens[i]].tobytes())
+        assert sp[i].tobytes() == osig, f"item {i}"

@@ -122,8 +122,9 @@ def test_serve_refusals(engine_factory):
 
 
 def test_serve_poisson_load(engine_factory):
-    """Open-loop Poisson arrivals: everything offered completes, in order of
-    magnitude of the batch latency."""
+    """Open-loop Poisson arrivals at 200k signatures/s (about 3% of the batch
+    throughput): everything offered completes, and the tail stays within a
+    few batch times (measured p99 about 10 ms)."""
     import bench
     from cumayo_amd.serve import Server
     eng = engine_factory(1)
@@ -135,4 +136,4 @@ def test_serve_poisson_load(engine_factory):
     print(f"[serve] poisson {rep}", flush=True)
     assert rep["refused"] == 0
     assert rep["completed"] == rep["submitted"] > 50_000
-    assert 0 < rep["p50_ms"] <= rep["p99_ms"] < 200
+    assert 0 < rep["p50_ms"] <= rep["p99_ms"] < 100
