@@ -18,6 +18,8 @@
 //         pivot columns, identical to the reference's echelon form plus
 //         back-substitution because that solution is unique.
 #pragma once
+#include <type_traits>
+
 #include "keccak_wave.hpp"
 #include "mayo_kernels.hpp"
 
@@ -82,14 +84,17 @@ struct Shared {
     uint8_t vr[VR_BYTES];
     uint64_t scol[N];            // scol[c] nibble j = S_j[c]
     uint32_t Mv[K * O * MW];     // M_i column x as an m-vector
-    uint32_t U[PAIRS * MW];
+    union {
+        uint32_t U[PAIRS * MW];       // pair sums (quadratic form, then y)
+        uint32_t Mb[KO * 4 * NG];     // bit planes of the M_j[x] (columns of A)
+    };
     uint32_t yu[UW];             // unreduced y
     uint32_t y[MW];              // y = t + sum E^l u_l
     uint32_t t[MW];
     uint32_t Ar[4 * NG];         // A r, bitsliced
     uint32_t yplanes[4 * NG];    // final y' column, bitsliced
     int16_t prow[KO];            // pivot row of each column, -1 if free
-    uint8_t pval[KO];            // product of the pivot values before each column
+    uint8_t pinv[KO];            // inverse of each column's pivot value
     uint8_t x[KO];
     uint8_t mul[256];            // GF(16) product table, a*16 + b
     uint8_t hbuf[DIGEST + 2 * SK + 1];  // hash input staging
@@ -437,39 +442,163 @@ __device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
 }
 
 // GF(16) product of a per-lane nibble and a uniform nibble via the LDS table.
-__device__ __forceinline__ uint32_t tmul(const Shared &S, uint32_t a, uint32_t b) {
+template <class LDS>
+__device__ __forceinline__ uint32_t tmul(const LDS &S, uint32_t a, uint32_t b) {
     return S.mul[(a << 4) | b];
 }
 
-// Build the A columns this lane owns (bitsliced) and the y' column.
-// Returns in cols[NCL][4][NG]; columns past KO (other than y') are zero.
-__device__ __forceinline__ void build_columns(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
+// LDS of the solve kernel: the bit planes of M, y and the elimination
+// bookkeeping (the signing-loop kernel uses the same fields of Shared).
+struct SolveShared {
+    uint8_t vr[VR_BYTES];
+    uint32_t Mb[KO * 4 * NG];
+    uint32_t y[MW];
+    uint32_t Ar[4 * NG];
+    uint32_t yplanes[4 * NG];
+    int16_t prow[KO];
+    uint8_t pinv[KO];
+    uint8_t x[KO];
+    uint8_t mul[256];
+};
+
+// Bit planes of the K*O nibble-packed vectors M_j[x] (MW words each, vector
+// v = j*O + x at src + v*MW, global or LDS): Mb[(v*4 + d)*NG + g] holds bit d
+// of rows 32g .. 32g+31.
+template <class SRC>
+__device__ __forceinline__ void planes_of_M(const SRC *src, uint32_t *Mb) {
+    for (int it = threadIdx.x; it < KO * NG; it += 64) {
+        const int v = it / NG, g = it % NG;
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = (4 * g + q < MW) ? src[v * MW + 4 * g + q] : 0u;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            uint32_t o = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o |= nib_plane8(w[q], d) << (8 * q);
+            Mb[(v * 4 + d) * NG + g] = o;
+        }
+    }
+}
+
+// Words of an unreduced column (rows < m + pairs - 1), the largest word
+// offset of a shift z^l (l < pairs), and the words of the rows >= m.
+constexpr int NGX = (M + PAIRS - 1 + 31) / 32;
+constexpr int WSX = (PAIRS - 1) / 32;
+constexpr int NOV = (PAIRS - 1 + 31) / 32;
+
+// Constant GF(16) multiple F * p of a bitsliced vector (4 planes).
+template <uint32_t F>
+__device__ __forceinline__ void bs_const_mul(const uint32_t (&p)[4], uint32_t (&o)[4]) {
+    uint32_t q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+    o[0] = o[1] = o[2] = o[3] = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        if ((F >> b) & 1) { o[0] ^= q0; o[1] ^= q1; o[2] ^= q2; o[3] ^= q3; }
+        bs_mul_a(q0, q1, q2, q3);
+    }
+}
+
+// Column c = (block b, x) of A (sign.cu:333-393), bitsliced over its m rows:
+// sum over the pairs of block b of z^l M_j[x], the z^l being row shifts
+// (per lane, since b is), accumulated over m + pairs - 1 rows and reduced
+// once: row m + t folds into rows t .. t+3 as z^t (F0 + F1 z + F2 z^2 + F3 z^3).
+template <class LDS>
+__device__ __forceinline__ void build_columns(const LDS &S, uint32_t (&cols)[NCL][4][NG]) {
     const int lane = threadIdx.x;
 #pragma unroll
     for (int s = 0; s < NCL; ++s) {
         const int c = col_of(s, lane);
-        uint32_t v[MW];
 #pragma unroll
-        for (int q = 0; q < MW; ++q) v[q] = 0;
+        for (int d = 0; d < 4; ++d)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) cols[s][d][g] = 0;
         if (c < KO) {
             const int b = c / O, x = c % O;
-            uint32_t acc[UW];
+            uint32_t acc[4][NGX];
 #pragma unroll
-            for (int q = 0; q < UW; ++q) acc[q] = 0;
-            // pairs (b, j), j >= b: block b gets z^l M_j; pairs (i, b), i < b: z^l M_i
-            for (int j = b; j < K; ++j) add_shift(acc, &S.Mv[(j * O + x) * MW], ell_of(b, j));
-            for (int i = 0; i < b; ++i) add_shift(acc, &S.Mv[(i * O + x) * MW], ell_of(i, b));
-            reduce_unreduced(acc, v);
+            for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int q = 0; q < NGX; ++q) acc[d][q] = 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const int ell = j >= b ? ell_of(b, j) : ell_of(j, b);
+                const int ws = ell >> 5;
+                const uint32_t ns = (uint32_t)ell & 31u;
+                uint32_t e[WSX + 1];
+#pragma unroll
+                for (int t = 0; t <= WSX; ++t) e[t] = ws == t ? 0xFFFFFFFFu : 0u;
+                const uint32_t *src = &S.Mb[(j * O + x) * 4 * NG];
+#pragma unroll
+                for (int d = 0; d < 4; ++d)
+#pragma unroll
+                    for (int i = 0; i < NG; ++i) {
+                        const uint64_t sh = (uint64_t)src[d * NG + i] << ns;
+                        const uint32_t lo = (uint32_t)sh, hi = (uint32_t)(sh >> 32);
+#pragma unroll
+                        for (int t = 0; t <= WSX; ++t) {
+                            if (i + t < NGX) acc[d][i + t] = gf16::axm(acc[d][i + t], lo, e[t]);
+                            if (i + t + 1 < NGX) acc[d][i + t + 1] = gf16::axm(acc[d][i + t + 1], hi, e[t]);
+                        }
+                    }
+            }
+            // rows >= m: ov row t = acc row m + t
+            uint32_t ov[NOV][4];
+#pragma unroll
+            for (int k = 0; k < NOV; ++k)
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    const int q = M / 32 + k;
+                    const uint32_t lo = q < NGX ? acc[d][q] : 0u, hi = q + 1 < NGX ? acc[d][q + 1] : 0u;
+                    ov[k][d] = (M % 32) ? __builtin_amdgcn_alignbit(hi, lo, M % 32) : lo;
+                }
+            // fold: rows t + i += F_i * ov row t (t + i < m)
+            auto fold = [&](auto Fc, int i) {
+                constexpr uint32_t F = decltype(Fc)::value;
+                if constexpr (F != 0) {
+                    uint32_t prev[4] = {0, 0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k <= NOV; ++k) {
+                        uint32_t pr[4] = {0, 0, 0, 0};
+                        if (k < NOV) bs_const_mul<F>(ov[k], pr);
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            const uint32_t v = i ? __builtin_amdgcn_alignbit(pr[d], prev[d], 32 - i) : pr[d];
+                            if (k < NGX) acc[d][k] ^= v;
+                            prev[d] = pr[d];
+                        }
+                    }
+                }
+            };
+            fold(std::integral_constant<uint32_t, F0>{}, 0);
+            fold(std::integral_constant<uint32_t, F1>{}, 1);
+            fold(std::integral_constant<uint32_t, F2>{}, 2);
+            fold(std::integral_constant<uint32_t, F3>{}, 3);
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    cols[s][d][g] = (g == NG - 1 && (M % 32)) ? acc[d][g] & ((1u << (M % 32)) - 1u) : acc[d][g];
         } else if (c == KO) {
+            uint32_t v[MW];
 #pragma unroll
             for (int q = 0; q < MW; ++q) v[q] = S.y[q];
+            to_planes(v, cols[s]);
         }
-        to_planes(v, cols[s]);
     }
 }
 
-// Gauss-Jordan solve; x (S.x) = r + z. Returns 1 on full rank.
-__device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
+// Gauss-Jordan on the columns of [A | y - A r]; x (S.x) = r + z. Returns 1
+// on full rank. Each pivot step: the pivot column f is broadcast (SGPRs),
+// its pivot row pr is the lowest unused row with a nonzero entry, p = f[pr];
+// every column c becomes c + (c[pr] / p) f', with f' = f except f'[pr] = 0,
+// which clears column pc outside row pr and keeps row pr. The per-lane
+// scalar q = c[pr] / p takes 8 ops; the multiples of f' are uniform.
+// Pivot columns are the greedy ones in column order, as in the reference's
+// echelon form (sign.cu:10-145), and z on them is y'[pr] / p: the solution
+// with the free variables at zero is unique, so x equals the reference's.
+template <class LDS>
+__device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG]) {
     const int lane = threadIdx.x;
     const uint8_t *rv = S.vr + R_OFF;
     // y' = y - A r: every A column times its r nibble, XOR-reduced over lanes
@@ -509,11 +638,12 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
 #pragma unroll
             for (int g = 0; g < NG; ++g) cols[Y_SET][d][g] ^= S.Ar[d * NG + g];
     }
+    // GF(16) inverses as a 64-bit table, nibble a = a^-1 (0 -> 0)
+    constexpr uint64_t INV = 0x834A5C2F67BDE910ull;
     uint32_t used[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) used[g] = 0;
     int rank = 0;
-    uint32_t prefix = 1;
     // pivots in column order: set NCL-1 (columns < SHIFT) first, then 0, 1, ...
 #pragma unroll
     for (int ord = 0; ord < NCL; ++ord) {
@@ -523,90 +653,76 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
         for (int pl = 0; pl < nl; ++pl) {
             const int pc = col_of(ps, pl);
             if (pc >= KO) break;
-            // pivot row: lowest unused row with a nonzero entry in column pc
-            int pr = -1;
-#pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const uint32_t nz = __builtin_amdgcn_readlane(
-                    (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~used[g], pl);
-                if (pr < 0 && nz) pr = 32 * g + __builtin_ctz(nz);
-            }
-            const bool has = pr >= 0;
-            const int prr = has ? pr : 0, pg = prr >> 5, pb = prr & 31;
-            // broadcast the pivot column f; p = f[pr]
             uint32_t f[4][NG];
 #pragma unroll
             for (int d = 0; d < 4; ++d)
 #pragma unroll
                 for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
-            uint32_t p = 0;
+            int pr = -1;
 #pragma unroll
-            for (int g = 0; g < NG; ++g)
-                if (g == pg) {
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) p |= ((f[d][g] >> pb) & 1u) << d;
-                }
-            // Fraction-free step, no inverse: every column c becomes
-            // p * c + c[pr] * f', with f' = f except f'[pr] = p + 1. Rows
-            // a != pr: p c[a] + c[pr] f[a] (column pc turns zero there); row
-            // pr keeps c[pr]. A free column (no pivot) takes p = 1, f' = 0.
-#pragma unroll
-            for (int g = 0; g < NG; ++g) f[0][g] ^= (has && g == pg) ? (1u << pb) : 0u;
-            const uint32_t pu = has ? p : 1u;
-            const uint32_t pm0 = gf16::bitmask(pu, 0), pm1 = gf16::bitmask(pu, 1);
-            const uint32_t pm2 = gf16::bitmask(pu, 2), pm3 = gf16::bitmask(pu, 3);
-            const uint32_t fz = has ? 0xFFFFFFFFu : 0u;
-#pragma unroll
-            for (int s = 0; s < NCL; ++s) {
-                // sets whose columns all precede pc need no update (their
-                // pivot columns stay scaled unit vectors; free columns do not
-                // enter x), unless they hold y'
-                if (s != Y_SET && set_rank(s) < ord) continue;
-                uint32_t m[4];
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    uint32_t wsel = cols[s][d][0];
-#pragma unroll
-                    for (int g = 1; g < NG; ++g) wsel = (g == pg) ? cols[s][d][g] : wsel;
-                    m[d] = (uint32_t)__builtin_amdgcn_sbfe((int)wsel, (uint32_t)pb, 1u) & fz;
-                }
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    // p * c: multiples of the lane's own column, uniform masks
-                    uint32_t c0 = cols[s][0][g], c1 = cols[s][1][g], c2 = cols[s][2][g], c3 = cols[s][3][g];
-                    uint32_t o0 = c0 & pm0, o1 = c1 & pm0, o2 = c2 & pm0, o3 = c3 & pm0;
-                    bs_mul_a(c0, c1, c2, c3);
-                    o0 = gf16::axm(o0, c0, pm1); o1 = gf16::axm(o1, c1, pm1);
-                    o2 = gf16::axm(o2, c2, pm1); o3 = gf16::axm(o3, c3, pm1);
-                    bs_mul_a(c0, c1, c2, c3);
-                    o0 = gf16::axm(o0, c0, pm2); o1 = gf16::axm(o1, c1, pm2);
-                    o2 = gf16::axm(o2, c2, pm2); o3 = gf16::axm(o3, c3, pm2);
-                    bs_mul_a(c0, c1, c2, c3);
-                    o0 = gf16::axm(o0, c0, pm3); o1 = gf16::axm(o1, c1, pm3);
-                    o2 = gf16::axm(o2, c2, pm3); o3 = gf16::axm(o3, c3, pm3);
-                    // + c[pr] * f': uniform multiples of f', per-lane masks
-                    uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        o0 = gf16::axm(o0, a0, m[b]); o1 = gf16::axm(o1, a1, m[b]);
-                        o2 = gf16::axm(o2, a2, m[b]); o3 = gf16::axm(o3, a3, m[b]);
-                        bs_mul_a(a0, a1, a2, a3);
-                    }
-                    cols[s][0][g] = o0; cols[s][1][g] = o1; cols[s][2][g] = o2; cols[s][3][g] = o3;
-                }
+            for (int g = NG - 1; g >= 0; --g) {
+                const uint32_t nz = (f[0][g] | f[1][g] | f[2][g] | f[3][g]) & ~used[g];
+                if (nz) pr = 32 * g + __builtin_ctz(nz);
             }
-            // product of the earlier pivots: z at this column is later
-            // y'[pr] * prefix / (product of all pivots)
-            S.pval[pc] = (uint8_t)prefix;
-            if (has) prefix = gf16::mul(prefix, p);
+            if (pr >= 0) {
+                const int pg = pr >> 5, pb = pr & 31;
+                uint32_t p = 0;
 #pragma unroll
-            for (int g = 0; g < NG; ++g)
-                if (has && g == pg) used[g] |= 1u << pb;
-            rank += has ? 1 : 0;
+                for (int g = 0; g < NG; ++g)
+                    if (g == pg) {
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            p |= ((f[d][g] >> pb) & 1u) << d;
+                            f[d][g] &= ~(1u << pb);
+                        }
+                    }
+                const uint32_t ip = (uint32_t)(INV >> (4 * p)) & 0xFu;
+                // a^b * p^-1 for the four bits of c[pr]
+                uint32_t im[4];
+                im[0] = ip;
+#pragma unroll
+                for (int b = 1; b < 4; ++b) im[b] = gf16::mul_a(im[b - 1]) & 0xFu;
+#pragma unroll
+                for (int s = 0; s < NCL; ++s) {
+                    // sets whose columns all precede pc need no update (pivot
+                    // columns are p e_pr, which the later steps leave alone;
+                    // free columns do not enter x), unless they hold y'
+                    if (s != Y_SET && set_rank(s) < ord) continue;
+                    uint32_t q = 0;
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        uint32_t wsel = cols[s][d][0];
+#pragma unroll
+                        for (int g = 1; g < NG; ++g) wsel = (g == pg) ? cols[s][d][g] : wsel;
+                        const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)wsel, (uint32_t)pb, 1u);
+                        q = gf16::axm(q, m, im[d]);
+                    }
+                    uint32_t qm[4];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) qm[b] = gf16::bitmask(q, b);
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
+                        uint32_t c0 = cols[s][0][g], c1 = cols[s][1][g], c2 = cols[s][2][g], c3 = cols[s][3][g];
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            c0 = gf16::axm(c0, a0, qm[b]); c1 = gf16::axm(c1, a1, qm[b]);
+                            c2 = gf16::axm(c2, a2, qm[b]); c3 = gf16::axm(c3, a3, qm[b]);
+                            bs_mul_a(a0, a1, a2, a3);
+                        }
+                        cols[s][0][g] = c0; cols[s][1][g] = c1; cols[s][2][g] = c2; cols[s][3][g] = c3;
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    if (g == pg) used[g] |= 1u << pb;
+                ++rank;
+                S.pinv[pc] = (uint8_t)ip;
+            }
             S.prow[pc] = (int16_t)pr;  // same value from every lane
         }
     }
-    // x = r, plus y'[pivot row] on the pivot columns
+    // x = r, plus y'[pivot row] / p on the pivot columns
     if (lane == Y_LANE) {
 #pragma unroll
         for (int d = 0; d < 4; ++d)
@@ -615,7 +731,6 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
     }
     __syncthreads();
     const int ok = rank == M;
-    const uint32_t tinv = gf16::inv(prefix);
     for (int c = lane; c < KO; c += 64) {
         uint32_t xv = mvec::nib(rv, c);
         const int pr = S.prow[c];
@@ -623,7 +738,7 @@ __device__ int solve(Shared &S, uint32_t (&cols)[NCL][4][NG]) {
             uint32_t yn = 0;
 #pragma unroll
             for (int d = 0; d < 4; ++d) yn |= ((S.yplanes[d * NG + (pr >> 5)] >> (pr & 31)) & 1u) << d;
-            xv ^= tmul(S, tmul(S, yn, S.pval[c]), tinv);
+            xv ^= tmul(S, yn, S.pinv[c]);
         }
         S.x[c] = (uint8_t)xv;
     }
@@ -640,7 +755,8 @@ __device__ __forceinline__ void wave_scol(Shared &S) {
 }
 
 // s_i = (V_i + O x_i || x_i) and the salt (sign.cu:147-191, 472-477)
-__device__ __forceinline__ void wave_write_sig(const Shared &S, const SignItem &it) {
+template <class LDS>
+__device__ __forceinline__ void wave_write_sig(const LDS &S, const SignItem &it) {
     const int lane = threadIdx.x;
     const uint8_t *Ob = it.esk + SK;
     for (int b = lane; b < SIG_S_BYTES; b += 64) {
@@ -698,6 +814,8 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int
         quad_form<KL, O, false>(it.esk, S.scol, S.U, S.Mv);
         __syncthreads();
         wave_y(S, S.t);
+        planes_of_M(S.Mv, S.Mb);  // Mb overlays U, which wave_y has consumed
+        __syncthreads();
         uint32_t cols[NCL][4][NG];
         build_columns(S, cols);
         ok = solve(S, cols);
@@ -735,12 +853,11 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
 template <class Acc>
 __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n) {
-    __shared__ Shared S;
+    __shared__ SolveShared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
-    const uint32_t *mv = (const uint32_t *)(it.scratch + SCR_MV);
-    for (int e = lane; e < K * O * MW; e += 64) S.Mv[e] = mv[e];
+    planes_of_M((const uint32_t *)(it.scratch + SCR_MV), S.Mb);
     const uint32_t *yv = (const uint32_t *)(it.scratch + SCR_Y);
     for (int e = lane; e < MW; e += 64) S.y[e] = yv[e];
     for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
