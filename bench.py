@@ -73,6 +73,9 @@ def main():
                     help="offered Poisson rate as a fraction of the measured batch throughput")
     ap.add_argument("--serve-batch", type=int, default=16384, help="requests per serve slot")
     ap.add_argument("--serve-slots", type=int, default=3, help="serve batches in flight")
+    ap.add_argument("--flow-batch", type=int, default=20000,
+                    help="batch of the reference-flow program (keypair, signWithesk, "
+                         "verifyWithepk over 10 streams, 32-byte messages); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -278,6 +281,20 @@ def main():
         out["serve"] = serve
         out["serve_p50_ms"] = serve["sign"]["p50_ms"]
         out["serve_p99_ms"] = serve["sign"]["p99_ms"]
+
+    if args.flow_batch > 0:
+        # the reference's own benchmark method through the drop-in C ABI
+        # (a C++ program linked against the library): per-item key expansion
+        # and both PCIe copies inside the timed region
+        import subprocess
+        from cumayo_amd import build as _build
+        print(f"[bench] reference flow, batch {args.flow_batch}", file=sys.stderr, flush=True)
+        res = subprocess.run([_build.flow_path(v), str(args.flow_batch), "10", "10", "32"],
+                             capture_output=True, text=True, timeout=300)
+        if res.returncode == 0:
+            out["reference_flow"] = json.loads(res.stdout.strip().splitlines()[-1])
+        else:
+            out["reference_flow"] = {"error": (res.stdout + res.stderr)[-300:]}
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O

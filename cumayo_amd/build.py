@@ -40,9 +40,32 @@ def build_one(version: int, extra=()) -> str:
     return out
 
 
+FLOW_SRC = os.path.join(HERE, "csrc", "bench_flow.cpp")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def flow_path(version: int) -> str:
+    return os.path.join(LIBDIR, f"bench_flow_mayo{version}")
+
+
+def build_flow(version: int) -> str:
+    """The reference-flow benchmark program, linked against the library."""
+    out = flow_path(version)
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", FLOW_SRC, f"-L{LIBDIR}",
+           f"-lcumayo_mayo{version}", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for bench_flow MAYO_{version}:\n{res.stderr[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(extra=()) -> list:
+    def one(v):
+        lib = build_one(v, extra)
+        return [lib, build_flow(v)]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        return list(ex.map(lambda v: build_one(v, extra), VERSIONS))
+        return [p for pair in ex.map(one, VERSIONS) for p in pair]
 
 
 if __name__ == "__main__":

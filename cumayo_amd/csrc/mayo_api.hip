@@ -67,9 +67,56 @@ void ensure_tables() {
 
 inline unsigned blocks64(size_t n) { return (unsigned)((n + 63) / 64); }
 
+// Rows of `width` bytes from src (pitch spitch) to dst (pitch dpitch); one
+// workgroup per group of rows, dwords when every address and pitch allows.
+template <class T>
+__global__ void __launch_bounds__(256) k_copy_rows(uint8_t *__restrict__ dst, size_t dpitch,
+                                                   const uint8_t *__restrict__ src, size_t spitch,
+                                                   size_t width, size_t rows, size_t rows_per_block) {
+    const size_t r0 = (size_t)blockIdx.x * rows_per_block;
+    const size_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+    const size_t wn = width / sizeof(T);
+    for (size_t r = r0; r < r1; ++r) {
+        const T *s = (const T *)(src + r * spitch);
+        T *d = (T *)(dst + r * dpitch);
+        for (size_t b = threadIdx.x; b < wn; b += 256) d[b] = s[b];
+    }
+}
+
+// Address at which the GPU reaches a host buffer: the buffer itself when it
+// is pinned (page-locked and mapped at the same address), else null.
+const void *device_view(const void *host) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, host) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory: not an error of the caller's
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || !a.devicePointer || a.devicePointer != a.hostPointer) return nullptr;
+    return host;
+}
+
+// Pitched host <-> device copies of the entry points. hipMemcpy2DAsync moves
+// one row per DMA transfer, which costs microseconds per row for the short
+// rows of these pools; from pinned host memory a copy kernel reads or writes
+// the host rows over PCIe directly instead. Pageable host memory goes
+// through hipMemcpy2DAsync.
 void copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows,
             hipMemcpyKind kind, hipStream_t stream) {
     if (!rows || !width) return;
+    const void *host = kind == hipMemcpyHostToDevice ? src : (const void *)dst;
+    if (device_view(host)) {
+        const size_t rpb = rows >= 4096 ? (rows + 4095) / 4096 : 1;
+        const unsigned grid = (unsigned)((rows + rpb - 1) / rpb);
+        const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | dpitch | spitch | width;
+        if ((al & 3) == 0)
+            k_copy_rows<uint32_t><<<grid, 256, 0, stream>>>((uint8_t *)dst, dpitch, (const uint8_t *)src, spitch,
+                                                             width, rows, rpb);
+        else
+            k_copy_rows<uint8_t><<<grid, 256, 0, stream>>>((uint8_t *)dst, dpitch, (const uint8_t *)src, spitch,
+                                                            width, rows, rpb);
+        note_launch();
+        return;
+    }
     note(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, stream));
 }
 

@@ -8,6 +8,7 @@ import hashlib
 import numpy as np
 import pytest
 
+from cumayo_amd.engine import a128
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -242,3 +243,39 @@ def test_prepared_tables_match_reference_layout(engine_factory, v):
     for i in list(range(0, n, 37)) + [0, 1]:
         osig, _ = O.sign(v, esk_h[kidx[i]].tobytes(), flat[offs[i]:offs[i] + lens[i]].tobytes())
         assert sp[i].tobytes() == osig, f"item {i}"
+
+
+def test_pitched_copies_pinned_and_pageable(engine_factory):
+    """The HtD / DtH entry points give the same rows from pinned host memory
+    (copy kernel) and from pageable host memory (hipMemcpy2DAsync)."""
+    import torch
+    v = 1
+    eng = engine_factory(v)
+    z = eng.sizes
+    batch = 37
+    row = a128(z.sk_bytes) + a128(z.pk_bytes) + a128(z.p_bytes)
+    pool, pitch = eng.pool(batch, row)
+    st = eng.stream_handle()
+    eng.lib.crypto_sign_keypair(pool.data_ptr(), pitch, batch, st, 3)
+    pk_pin, sk_pin = eng.pinned((batch, z.pk_bytes)), eng.pinned((batch, z.sk_bytes))
+    pk_pag = np.zeros((batch, z.pk_bytes), np.uint8)
+    sk_pag = np.zeros((batch, z.sk_bytes), np.uint8)
+    eng.lib.crypto_sign_keypair_DtH(pk_pin.data_ptr(), sk_pin.data_ptr(), pool.data_ptr(), pitch, batch, st)
+    eng.lib.crypto_sign_keypair_DtH(pk_pag.ctypes.data, sk_pag.ctypes.data, pool.data_ptr(), pitch, batch, st)
+    torch.cuda.synchronize()
+    eng.check()
+    assert np.array_equal(pk_pin.numpy(), pk_pag) and np.array_equal(sk_pin.numpy(), sk_pag)
+    opk, osk = O.keypair(v, 3, batch - 1)
+    assert pk_pag[-1].tobytes() == opk and sk_pag[-1].tobytes() == osk
+    # upload the compact keys both ways and expand: identical esk rows
+    outs = []
+    for src in (eng.pinned(sk_pag.shape, sk_pag), sk_pag):
+        ep, epitch = eng.pool(batch, z.esk_bytes)
+        ptr = src.data_ptr() if hasattr(src, "data_ptr") else src.ctypes.data
+        eng.lib.crypto_sign_expandsk_HtD(ptr, ep.data_ptr(), epitch, batch, st)
+        eng.lib.crypto_sign_expandsk(ep.data_ptr(), epitch, batch, st)
+        torch.cuda.synchronize()
+        outs.append(ep[:, :z.esk_bytes].cpu().numpy())
+    eng.check()
+    assert np.array_equal(outs[0], outs[1])
+    assert outs[0][-1].tobytes() == O.expandsk(v, osk)
