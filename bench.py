@@ -38,6 +38,11 @@ VERIFY_MACS = {1: 4_197_180, 2: 998_656, 3: 11_294_316, 5: 26_389_848}
 # MACs per 16 bitsliced AND-XOR ops -> 2 MACs per op.
 PEAK_GF16_TMACS = 256 * 64 * 2.4e9 * 2 / 1e12
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the signing linear-algebra pass, measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.00 and x1.00 by the
+# same counters over a 256 MiB copy) on the default workload; the run is
+# summarised in profiles/sign_verify_mayo1_65536x64keys.md
+STAGE2_TRAFFIC = {(1, 65536, 64): 576_742_707}
 
 
 def make_workload(n: int, nkeys: int, lo: int, hi: int, tag: str = "cfg2"):
@@ -250,7 +255,8 @@ def main():
             "macs_per_sign": SIGN_MACS[v],
             "hbm_algorithmic_GBs": round(algo_bytes / (main_ms * 1e-3) / 1e9, 2),
             "hbm_peak_GBs": HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": STAGE2_TRAFFIC.get((v, n, args.keys)),
+            "traffic_unit": "bytes per launch (PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
         },
     }
 
