@@ -25,55 +25,77 @@ __constant__ uint64_t KECCAK_RC[24] = {
     0x000000000000800aull, 0x800000008000000aull, 0x8000000080008081ull,
     0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
 
+// 64-bit rotation by a constant on (lo, hi) halves: two v_alignbit_b32.
+template <int R>
+__device__ __forceinline__ void rot64(uint32_t lo, uint32_t hi, uint32_t &olo, uint32_t &ohi) {
+    if constexpr (R == 0) {
+        olo = lo; ohi = hi;
+    } else if constexpr (R < 32) {
+        ohi = __builtin_amdgcn_alignbit(hi, lo, 32 - R);
+        olo = __builtin_amdgcn_alignbit(lo, hi, 32 - R);
+    } else if constexpr (R == 32) {
+        olo = hi; ohi = lo;
+    } else {
+        ohi = __builtin_amdgcn_alignbit(lo, hi, 64 - R);
+        olo = __builtin_amdgcn_alignbit(hi, lo, 64 - R);
+    }
+}
+
+// a ^ (~b & c) in one v_bitop3_b32 (truth table 0xD2 over (a, b, c)).
+__device__ __forceinline__ uint32_t chi(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);
+}
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// Keccak-f[1600] on the state as 32-bit halves: theta with 3-input XORs
+// (bitop3), rho as alignbit pairs, pi by renaming, chi as one bitop3 per half.
 __device__ __forceinline__ void f1600(uint64_t *A) {
+    uint32_t L[25], H[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) { L[i] = (uint32_t)A[i]; H[i] = (uint32_t)(A[i] >> 32); }
 #pragma unroll 1
     for (int round = 0; round < 24; ++round) {
-        uint64_t C0 = A[0] ^ A[5] ^ A[10] ^ A[15] ^ A[20];
-        uint64_t C1 = A[1] ^ A[6] ^ A[11] ^ A[16] ^ A[21];
-        uint64_t C2 = A[2] ^ A[7] ^ A[12] ^ A[17] ^ A[22];
-        uint64_t C3 = A[3] ^ A[8] ^ A[13] ^ A[18] ^ A[23];
-        uint64_t C4 = A[4] ^ A[9] ^ A[14] ^ A[19] ^ A[24];
-        uint64_t D0 = C4 ^ rotl(C1, 1), D1 = C0 ^ rotl(C2, 1), D2 = C1 ^ rotl(C3, 1);
-        uint64_t D3 = C2 ^ rotl(C4, 1), D4 = C3 ^ rotl(C0, 1);
-        // theta, then rho and pi: B[y + 5*((2x+3y)%5)] = rotl(A[x+5y], r[x][y])
-        uint64_t B0 = A[0] ^ D0;
-        uint64_t B10 = rotl(A[1] ^ D1, 1);
-        uint64_t B20 = rotl(A[2] ^ D2, 62);
-        uint64_t B5 = rotl(A[3] ^ D3, 28);
-        uint64_t B15 = rotl(A[4] ^ D4, 27);
-        uint64_t B16 = rotl(A[5] ^ D0, 36);
-        uint64_t B1 = rotl(A[6] ^ D1, 44);
-        uint64_t B11 = rotl(A[7] ^ D2, 6);
-        uint64_t B21 = rotl(A[8] ^ D3, 55);
-        uint64_t B6 = rotl(A[9] ^ D4, 20);
-        uint64_t B7 = rotl(A[10] ^ D0, 3);
-        uint64_t B17 = rotl(A[11] ^ D1, 10);
-        uint64_t B2 = rotl(A[12] ^ D2, 43);
-        uint64_t B12 = rotl(A[13] ^ D3, 25);
-        uint64_t B22 = rotl(A[14] ^ D4, 39);
-        uint64_t B23 = rotl(A[15] ^ D0, 41);
-        uint64_t B8 = rotl(A[16] ^ D1, 45);
-        uint64_t B18 = rotl(A[17] ^ D2, 15);
-        uint64_t B3 = rotl(A[18] ^ D3, 21);
-        uint64_t B13 = rotl(A[19] ^ D4, 8);
-        uint64_t B14 = rotl(A[20] ^ D0, 18);
-        uint64_t B24 = rotl(A[21] ^ D1, 2);
-        uint64_t B9 = rotl(A[22] ^ D2, 61);
-        uint64_t B19 = rotl(A[23] ^ D3, 56);
-        uint64_t B4 = rotl(A[24] ^ D4, 14);
+        uint32_t CL[5], CH[5], DL[5], DH[5];
+#pragma unroll
+        for (int x = 0; x < 5; ++x) {
+            CL[x] = xor3(xor3(L[x], L[x + 5], L[x + 10]), L[x + 15], L[x + 20]);
+            CH[x] = xor3(xor3(H[x], H[x + 5], H[x + 10]), H[x + 15], H[x + 20]);
+        }
+#pragma unroll
+        for (int x = 0; x < 5; ++x) {
+            uint32_t rl, rh;
+            rot64<1>(CL[(x + 1) % 5], CH[(x + 1) % 5], rl, rh);
+            DL[x] = CL[(x + 4) % 5] ^ rl;
+            DH[x] = CH[(x + 4) % 5] ^ rh;
+        }
+        // theta, then rho and pi: B[y + 5*((2x+3y)%5)] = rotl(A[x+5y] ^ D[x], r[x][y])
+        uint32_t BL[25], BH[25];
+#define KECCAK_RP(src, dst, r) rot64<r>(L[src] ^ DL[(src) % 5], H[src] ^ DH[(src) % 5], BL[dst], BH[dst])
+        KECCAK_RP(0, 0, 0);   KECCAK_RP(1, 10, 1);  KECCAK_RP(2, 20, 62); KECCAK_RP(3, 5, 28);
+        KECCAK_RP(4, 15, 27); KECCAK_RP(5, 16, 36); KECCAK_RP(6, 1, 44);  KECCAK_RP(7, 11, 6);
+        KECCAK_RP(8, 21, 55); KECCAK_RP(9, 6, 20);  KECCAK_RP(10, 7, 3);  KECCAK_RP(11, 17, 10);
+        KECCAK_RP(12, 2, 43); KECCAK_RP(13, 12, 25); KECCAK_RP(14, 22, 39); KECCAK_RP(15, 23, 41);
+        KECCAK_RP(16, 8, 45); KECCAK_RP(17, 18, 15); KECCAK_RP(18, 3, 21); KECCAK_RP(19, 13, 8);
+        KECCAK_RP(20, 14, 18); KECCAK_RP(21, 24, 2); KECCAK_RP(22, 9, 61); KECCAK_RP(23, 19, 56);
+        KECCAK_RP(24, 4, 14);
+#undef KECCAK_RP
         // chi
-        A[0] = B0 ^ (~B1 & B2);   A[1] = B1 ^ (~B2 & B3);   A[2] = B2 ^ (~B3 & B4);
-        A[3] = B3 ^ (~B4 & B0);   A[4] = B4 ^ (~B0 & B1);
-        A[5] = B5 ^ (~B6 & B7);   A[6] = B6 ^ (~B7 & B8);   A[7] = B7 ^ (~B8 & B9);
-        A[8] = B8 ^ (~B9 & B5);   A[9] = B9 ^ (~B5 & B6);
-        A[10] = B10 ^ (~B11 & B12); A[11] = B11 ^ (~B12 & B13); A[12] = B12 ^ (~B13 & B14);
-        A[13] = B13 ^ (~B14 & B10); A[14] = B14 ^ (~B10 & B11);
-        A[15] = B15 ^ (~B16 & B17); A[16] = B16 ^ (~B17 & B18); A[17] = B17 ^ (~B18 & B19);
-        A[18] = B18 ^ (~B19 & B15); A[19] = B19 ^ (~B15 & B16);
-        A[20] = B20 ^ (~B21 & B22); A[21] = B21 ^ (~B22 & B23); A[22] = B22 ^ (~B23 & B24);
-        A[23] = B23 ^ (~B24 & B20); A[24] = B24 ^ (~B20 & B21);
-        A[0] ^= KECCAK_RC[round];
+#pragma unroll
+        for (int y = 0; y < 25; y += 5)
+#pragma unroll
+            for (int x = 0; x < 5; ++x) {
+                L[y + x] = chi(BL[y + x], BL[y + (x + 1) % 5], BL[y + (x + 2) % 5]);
+                H[y + x] = chi(BH[y + x], BH[y + (x + 1) % 5], BH[y + (x + 2) % 5]);
+            }
+        const uint64_t rc = KECCAK_RC[round];
+        L[0] ^= (uint32_t)rc;
+        H[0] ^= (uint32_t)(rc >> 32);
     }
+#pragma unroll
+    for (int i = 0; i < 25; ++i) A[i] = (uint64_t)L[i] | ((uint64_t)H[i] << 32);
 }
 
 // Incremental SHAKE over byte pieces held anywhere the lane can address.
