@@ -44,7 +44,7 @@ constexpr int UW = MW + OVW;                   // words of an unreduced sum
 constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : (K <= 4 ? 2 : K <= 10 ? 2 : 1);
 constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : K <= 11 ? 3 : 2);
 // waves per SIMD of the solve kernel (the bitsliced columns of [A | y'])
-constexpr int SOLVE_OCC = M <= 80 ? 6 : 3;
+constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
 // column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
 // go to the last set so that only one set stays active for most pivots
 constexpr int SHIFT = NCOL - 64 * (NCL - 1);
