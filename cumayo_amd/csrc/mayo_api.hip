@@ -158,6 +158,40 @@ void launch_verify_pitched(uint8_t *pool, size_t pitch, size_t n, size_t mlen, h
     note_launch();
 }
 
+// NIST wire format (sm = signature || message), one wavefront per item.
+__global__ void __launch_bounds__(64) k_sm_pack(const uint8_t *__restrict__ sigs, const uint8_t *__restrict__ msgs,
+                                                const int64_t *__restrict__ msg_off,
+                                                const int32_t *__restrict__ msg_len, uint8_t *__restrict__ sm,
+                                                const int64_t *__restrict__ sm_off) {
+    const size_t i = blockIdx.x;
+    uint8_t *out = sm + sm_off[i];
+    const uint8_t *sg = sigs + i * (size_t)SIG_BYTES, *m = msgs + msg_off[i];
+    for (int b = threadIdx.x; b < SIG_BYTES; b += 64) out[b] = sg[b];
+    for (int b = threadIdx.x; b < msg_len[i]; b += 64) out[SIG_BYTES + b] = m[b];
+}
+
+__global__ void __launch_bounds__(64) k_sm_split(const uint8_t *__restrict__ sm, const int64_t *__restrict__ sm_off,
+                                                 const int32_t *__restrict__ sm_len, uint8_t *__restrict__ sigs,
+                                                 uint8_t *__restrict__ msgs, const int64_t *__restrict__ out_off,
+                                                 int32_t *__restrict__ len) {
+    const size_t i = blockIdx.x;
+    const int32_t total = sm_len[i];
+    const int32_t ml = total >= SIG_BYTES ? total - SIG_BYTES : 0;
+    const uint8_t *in = sm + sm_off[i];
+    uint8_t *sg = sigs + i * (size_t)SIG_BYTES, *m = msgs + out_off[i];
+    for (int b = threadIdx.x; b < SIG_BYTES; b += 64) sg[b] = total >= SIG_BYTES ? in[b] : 0;
+    for (int b = threadIdx.x; b < ml; b += 64) m[b] = in[SIG_BYTES + b];
+    if (threadIdx.x == 0) len[i] = ml;
+}
+
+__global__ void k_sm_finish(const int32_t *__restrict__ sm_len, const int32_t *__restrict__ len,
+                            uint8_t *__restrict__ results, int32_t *__restrict__ mlen_out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (sm_len[i] < SIG_BYTES) results[i] = 1;
+    if (mlen_out) mlen_out[i] = results[i] == 0 ? len[i] : -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -454,6 +488,48 @@ int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32
                         hipStream_t stream) {
     return cumayo_verify_table_stages(epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs,
                                       results, n, workspace, stream, 3);
+}
+
+size_t cumayo_crypto_sign_workspace_size(size_t n) {
+    return cumayo_sign_workspace_size(n) + n * (size_t)SIG_BYTES;
+}
+
+int cumayo_crypto_sign(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                       const int64_t *msg_off, const int32_t *msg_len, uint8_t *sm, const int64_t *sm_off,
+                       size_t n, uint8_t *workspace, hipStream_t stream) {
+    if (!n) return 0;
+    if (!sm || !sm_off) return (int)hipErrorInvalidValue;
+    uint8_t *sigs = workspace + cumayo_sign_workspace_size(n);
+    int e = cumayo_sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, workspace, stream, 3);
+    if (e) return e;
+    k_sm_pack<<<(unsigned)n, 64, 0, stream>>>(sigs, msgs, msg_off, msg_len, sm, sm_off);
+    const hipError_t le = hipGetLastError();
+    note(le);
+    return (int)le;
+}
+
+size_t cumayo_crypto_sign_open_workspace_size(size_t n) {
+    return cumayo_verify_workspace_size(n) + n * (sizeof(int32_t) + (size_t)SIG_BYTES);
+}
+
+int cumayo_crypto_sign_open(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *sm,
+                            const int64_t *sm_off, const int32_t *sm_len, uint8_t *msgs_out,
+                            const int64_t *out_off, int32_t *mlen_out, uint8_t *results, size_t n,
+                            uint8_t *workspace, hipStream_t stream) {
+    if (!n) return 0;
+    if (!sm || !sm_off || !sm_len || !msgs_out || !out_off || !results || !workspace)
+        return (int)hipErrorInvalidValue;
+    int32_t *len = (int32_t *)(workspace + cumayo_verify_workspace_size(n));
+    uint8_t *sigs = (uint8_t *)(len + n);
+    k_sm_split<<<(unsigned)n, 64, 0, stream>>>(sm, sm_off, sm_len, sigs, msgs_out, out_off, len);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) { note(le); return (int)le; }
+    int e = cumayo_verify_prepared(prepared, key_idx, msgs_out, out_off, len, sigs, results, n, workspace, stream, 3);
+    if (e) return e;
+    k_sm_finish<<<blocks64(n), 64, 0, stream>>>(sm_len, len, results, mlen_out, n);
+    le = hipGetLastError();
+    note(le);
+    return (int)le;
 }
 
 }  // extern "C"

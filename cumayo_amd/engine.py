@@ -107,6 +107,10 @@ EXPORTS = {
     "cumayo_prepare_epk": [_VP, _SZ, _SZ, _VP, _VP],
     "cumayo_sign_prepared": [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
     "cumayo_verify_prepared": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
+    "cumayo_crypto_sign_workspace_size": [_SZ],
+    "cumayo_crypto_sign": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
+    "cumayo_crypto_sign_open_workspace_size": [_SZ],
+    "cumayo_crypto_sign_open": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
 }
 _RESTYPES = {
     "cumayo_last_error": ctypes.c_int,
@@ -122,6 +126,10 @@ _RESTYPES = {
     "cumayo_prepare_epk": ctypes.c_int,
     "cumayo_sign_prepared": ctypes.c_int,
     "cumayo_verify_prepared": ctypes.c_int,
+    "cumayo_crypto_sign_workspace_size": _SZ,
+    "cumayo_crypto_sign": ctypes.c_int,
+    "cumayo_crypto_sign_open_workspace_size": _SZ,
+    "cumayo_crypto_sign_open": ctypes.c_int,
 }
 
 
@@ -440,3 +448,28 @@ class Engine:
             self.stream_handle(stream), stages)
         if err:
             raise RuntimeError(f"cumayo_verify_prepared failed: HIP error {err}")
+
+    # -- NIST wire format (sm = signature || message) -----------------------
+    def crypto_sign(self, prep_esk, key_idx, msgs, msg_off, msg_len, sm, sm_off, stream=None):
+        n = int(key_idx.numel())
+        ws = self.torch.empty(max(int(self.lib.cumayo_crypto_sign_workspace_size(n)), 1),
+                              dtype=self.torch.uint8, device=self.device)
+        err = self.lib.cumayo_crypto_sign(prep_esk.data_ptr(), key_idx.data_ptr(), msgs.data_ptr(),
+                                          msg_off.data_ptr(), msg_len.data_ptr(), sm.data_ptr(),
+                                          sm_off.data_ptr(), n, ws.data_ptr(), self.stream_handle(stream))
+        if err:
+            raise RuntimeError(f"cumayo_crypto_sign failed: HIP error {err}")
+        return ws
+
+    def crypto_sign_open(self, prep_epk, key_idx, sm, sm_off, sm_len, msgs_out, out_off, mlen_out,
+                         results, stream=None):
+        n = int(key_idx.numel())
+        ws = self.torch.empty(max(int(self.lib.cumayo_crypto_sign_open_workspace_size(n)), 1),
+                              dtype=self.torch.uint8, device=self.device)
+        err = self.lib.cumayo_crypto_sign_open(
+            prep_epk.data_ptr(), key_idx.data_ptr(), sm.data_ptr(), sm_off.data_ptr(), sm_len.data_ptr(),
+            msgs_out.data_ptr(), out_off.data_ptr(), mlen_out.data_ptr() if mlen_out is not None else None,
+            results.data_ptr(), n, ws.data_ptr(), self.stream_handle(stream))
+        if err:
+            raise RuntimeError(f"cumayo_crypto_sign_open failed: HIP error {err}")
+        return ws

@@ -172,6 +172,27 @@ int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, cons
                            uint8_t *results, size_t n, uint8_t *workspace, hipStream_t stream,
                            int stages);
 
+/* NIST wire format (the crypto_sign / crypto_sign_open convention of the
+ * MAYO submission: sm = signature || message), batched over prepared key
+ * tables; no counterpart in the reference, which exposes detached signatures
+ * only. Device pointers throughout; ragged offsets are caller-chosen
+ * (sm_off[i] = msg_off[i] + i * sig_bytes packs them densely).
+ * cumayo_crypto_sign writes sm_len = sig_bytes + msg_len[i] bytes at
+ * sm + sm_off[i]. cumayo_crypto_sign_open splits each sm into the signature
+ * and the message (copied to msgs_out + out_off[i]), verifies, and sets
+ * results[i] = 0 valid / 1 invalid (also for sm shorter than a signature)
+ * and mlen_out[i] = message length, or -1 when invalid (mlen_out may be
+ * NULL). */
+size_t cumayo_crypto_sign_workspace_size(size_t n);
+int cumayo_crypto_sign(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                       const int64_t *msg_off, const int32_t *msg_len, uint8_t *sm, const int64_t *sm_off,
+                       size_t n, uint8_t *workspace, hipStream_t stream);
+size_t cumayo_crypto_sign_open_workspace_size(size_t n);
+int cumayo_crypto_sign_open(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *sm,
+                            const int64_t *sm_off, const int32_t *sm_len, uint8_t *msgs_out,
+                            const int64_t *out_off, int32_t *mlen_out, uint8_t *results, size_t n,
+                            uint8_t *workspace, hipStream_t stream);
+
 /* ---------------- Part 3: continuous-batching server ---------------- */
 
 /* Requests queue on the host; a worker thread admits them into free batch
