@@ -28,6 +28,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process
+# (default 4); the serve runs keep up to 4 servers x 3 slot streams busy at
+# once, which on 4 queues serialise behind each other. Set before HIP starts.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 # Work per operation (GF(16) multiply-accumulates), from the reference loop
 # bounds: sign counts one attempt (sign.cu:256-437 plus the elimination),
@@ -61,6 +65,82 @@ def make_workload(n: int, nkeys: int, lo: int, hi: int, tag: str = "cfg2"):
     return np.frombuffer(bytes(flat), np.uint8), offs, lens, key_idx
 
 
+def log_uniform_workload(n: int, nkeys: int, lo: int, hi: int, tag: str):
+    """Like make_workload, lengths log-uniform in [lo, hi]: exp of a uniform
+    draw on [ln lo, ln hi] from the same SHAKE length stream."""
+    import math
+    lens = np.empty(n, np.int32)
+    for j in range(n):
+        r = int.from_bytes(hashlib.shake_256(f"mayo-bench/{tag}/len".encode() + j.to_bytes(8, "little")).digest(8), "little")
+        u = r / 2.0**64
+        lens[j] = min(hi, max(lo, int(round(math.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))))))
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    flat = bytearray(int(lens.sum()))
+    for j in range(n):
+        o = int(offs[j])
+        flat[o:o + int(lens[j])] = hashlib.shake_256(f"mayo-bench/{tag}/msg".encode() + j.to_bytes(8, "little")).digest(int(lens[j]))
+    return np.frombuffer(bytes(flat), np.uint8), offs, lens, (np.arange(n) % nkeys).astype(np.int32)
+
+
+def mixed_sets_run(eng1, args, weights=None, pool=8192, keys=64, loads=(0.25, 0.5, 0.75), seconds=None):
+    """Serve a mix of parameter sets on the one GPU: MAYO_1/2/3/5 = 40/20/20/20,
+    64 resident keys per set, message lengths log-uniform in [32 B, 16 KB],
+    half of the requests verifications, swept over offered loads given as
+    fractions of the mix's batch capacity (from each set's measured sign and
+    verify rates at a batch of `pool`)."""
+    import torch
+    from cumayo_amd import Engine
+    from cumayo_amd.mixed import MixedServer
+    weights = weights or {1: 0.4, 2: 0.2, 3: 0.2, 5: 0.2}
+    seconds = args.serve_seconds / 2 if seconds is None else seconds
+    engines, tables, pools, cap = {}, {}, {}, {}
+    for v in weights:
+        e = eng1 if v == eng1.version else Engine(v)
+        engines[v] = e
+        esk, epk = e.keypair_tables(keys, rand_index=0)
+        pesk, pepk = e.prepare_tables(esk, epk)
+        flat, offs, lens, kidx = log_uniform_workload(pool, keys, 32, 16384, tag=f"cfg5-{v}")
+        t = lambda a: torch.from_numpy(np.array(a)).to(e.device)
+        dk, dm, do, dl = t(kidx), t(flat), t(offs), t(lens)
+        sig = torch.zeros((pool, e.sizes.sig_bytes), dtype=torch.uint8, device=e.device)
+        res = torch.zeros(pool, dtype=torch.uint8, device=e.device)
+        ws_s, ws_v = e.sign_workspace(pool), e.verify_workspace(pool)
+        rates = []
+        for fn in (lambda: e.sign_prepared(pesk, dk, dm, do, dl, sig, ws_s),
+                   lambda: e.verify_prepared(pepk, dk, dm, do, dl, sig, res, ws_v)):
+            fn()
+            torch.cuda.synchronize(e.device)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize(e.device)
+            rates.append(3 * pool / (time.perf_counter() - t0))
+        cap[v] = rates
+        tables[v] = (esk, epk)
+        pools[v] = (flat, offs, lens, kidx, sig.cpu().numpy())
+    # mean GPU time per request of the mix (half signs, half verifications)
+    per_req = sum(w * (0.5 / cap[v][0] + 0.5 / cap[v][1]) for v, w in weights.items())
+    capacity = 1.0 / per_req
+    points = []
+    with MixedServer(engines, tables, max_batch=pool, n_slots=3, slot_msg_bytes=pool * 4096) as ms:
+        for load in loads:
+            print(f"[bench] serve mixed sets at {load:.0%} of {capacity:.0f}/s", file=sys.stderr, flush=True)
+            rep = ms.run_poisson(load * capacity, seconds, weights, 0.5, pools)
+            per_set = {k: {q: v[q] for q in ("offered_per_s", "achieved_per_s", "p50_ms", "p99_ms")}
+                       for k, v in rep.pop("per_set").items()}
+            points.append({"offered_frac_of_capacity": load,
+                           **{k: (round(x, 3) if isinstance(x, float) else x) for k, x in rep.items()},
+                           "per_set": per_set})
+    return {"mix": {f"MAYO_{v}": w for v, w in weights.items()},
+            "messages": "log-uniform [32 B, 16 KB], 50% verification, 64 keys per set",
+            "capacity_per_s": round(capacity, 1),
+            f"batch_rates_at_{pool}": {f"MAYO_{v}": {"sign_per_s": round(c[0], 1),
+                                                    "verify_per_s": round(c[1], 1)}
+                                       for v, c in cap.items()},
+            "vs_load": points}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +158,8 @@ def main():
                     help="offered Poisson rate as a fraction of the measured batch throughput")
     ap.add_argument("--serve-batch", type=int, default=16384, help="requests per serve slot")
     ap.add_argument("--serve-slots", type=int, default=3, help="serve batches in flight")
+    ap.add_argument("--no-mixed-sets", action="store_true",
+                    help="skip the mixed-parameter-set serve run (MAYO_1/2/3/5 = 40/20/20/20)")
     ap.add_argument("--flow-batch", type=int, default=20000,
                     help="batch of the reference-flow program (keypair, signWithesk, "
                          "verifyWithepk over 10 streams, 32-byte messages); 0 skips it")
@@ -268,22 +350,32 @@ def main():
         from cumayo_amd.serve import Server
         serve = {}
         mixed_rate = 1.0 / (0.5 / sign_per_s + 0.5 / verify_per_s)
+        sweep = sorted(set([0.3, 0.5, args.serve_load, 0.9]))
         with Server(eng, d_esk, d_epk, max_batch=args.serve_batch, n_slots=args.serve_slots,
                     slot_msg_bytes=args.serve_batch * 1024) as srv:
-            for name, frac_v, batch_rate in (("sign", 0.0, sign_per_s), ("mixed_50_50", 0.5, mixed_rate)):
-                rate = args.serve_load * batch_rate
-                print(f"[bench] serve {name}: Poisson {rate:.0f}/s for {args.serve_seconds} s",
-                      file=sys.stderr, flush=True)
-                rep = srv.run_poisson(rate, args.serve_seconds, frac_v, 17, flat, offs, lens, kidx, sigs)
-                serve[name] = {"offered_per_s": round(rep["offered_per_s"], 1),
-                               "achieved_per_s": round(rep["achieved_per_s"], 1),
-                               "offered_frac_of_batch": args.serve_load,
-                               "p50_ms": round(rep["p50_ms"], 3), "p90_ms": round(rep["p90_ms"], 3),
-                               "p99_ms": round(rep["p99_ms"], 3), "max_ms": round(rep["max_ms"], 3),
-                               "completed": rep["completed"], "refused": rep["refused"]}
+            def point(frac_v, rate, seconds):
+                rep = srv.run_poisson(rate, seconds, frac_v, 17, flat, offs, lens, kidx, sigs)
+                return {"offered_per_s": round(rep["offered_per_s"], 1),
+                        "achieved_per_s": round(rep["achieved_per_s"], 1),
+                        "p50_ms": round(rep["p50_ms"], 3), "p90_ms": round(rep["p90_ms"], 3),
+                        "p99_ms": round(rep["p99_ms"], 3), "max_ms": round(rep["max_ms"], 3),
+                        "completed": rep["completed"], "refused": rep["refused"]}
+            serve["sign_vs_load"] = []
+            for frac in sweep:
+                print(f"[bench] serve sign at {frac:.0%} of batch throughput", file=sys.stderr, flush=True)
+                secs = args.serve_seconds if frac == args.serve_load else args.serve_seconds / 2
+                serve["sign_vs_load"].append({"offered_frac_of_batch": frac,
+                                              **point(0.0, frac * sign_per_s, secs)})
+            serve["sign"] = next(x for x in serve["sign_vs_load"]
+                                 if x["offered_frac_of_batch"] == args.serve_load)
+            print("[bench] serve sign/verify 50/50", file=sys.stderr, flush=True)
+            serve["mixed_50_50"] = {"offered_frac_of_batch": args.serve_load,
+                                    **point(0.5, args.serve_load * mixed_rate, args.serve_seconds)}
             st = srv.stats()
         serve["slots"] = {"max_batch": args.serve_batch, "n_slots": args.serve_slots,
                           "batches": st["batches"], "overlapped_launches": st["overlapped_launches"]}
+        if not args.no_mixed_sets:
+            serve["mixed_sets"] = mixed_sets_run(eng, args)
         out["serve"] = serve
         out["serve_p50_ms"] = serve["sign"]["p50_ms"]
         out["serve_p99_ms"] = serve["sign"]["p99_ms"]

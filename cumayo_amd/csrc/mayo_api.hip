@@ -571,11 +571,12 @@ int64_t cumayo_serve_clock_ns(void) { return serve::now_ns(); }
 int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
                              double verify_fraction, uint64_t seed, const uint8_t *msg_pool,
                              const int64_t *msg_off, const int32_t *msg_len, const int32_t *msg_key,
-                             const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out) {
+                             const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out,
+                             int64_t *lat_out, size_t lat_cap) {
     if (!server || !out) return (int)hipErrorInvalidValue;
     return serve::run_poisson(*static_cast<serve::Server *>(server), rate_per_s, duration_s,
                               verify_fraction, seed, msg_pool, msg_off, msg_len, msg_key, sig_pool,
-                              pool_n, out);
+                              pool_n, out, lat_out, lat_cap);
 }
 
 }  // extern "C"

@@ -325,7 +325,7 @@ class Server {
 inline int run_poisson(Server &srv, double rate, double duration, double verify_fraction,
                        uint64_t seed, const uint8_t *msg_pool, const int64_t *msg_off,
                        const int32_t *msg_len, const int32_t *msg_key, const uint8_t *sig_pool,
-                       size_t pool_n, cumayo_serve_report *out) {
+                       size_t pool_n, cumayo_serve_report *out, int64_t *lat_out, size_t lat_cap) {
     if (rate <= 0 || duration <= 0 || !pool_n) return (int)hipErrorInvalidValue;
     std::mt19937_64 rng(seed);
     std::exponential_distribution<double> gap(rate);
@@ -384,6 +384,7 @@ inline int run_poisson(Server &srv, double rate, double duration, double verify_
     }
     drain();
     const int64_t t_done = now_ns();
+    if (lat_out) std::copy(lat.begin(), lat.begin() + (long)std::min(lat.size(), lat_cap), lat_out);
     std::sort(lat.begin(), lat.end());
     auto pct = [&](double q) {
         if (lat.empty()) return 0.0;

@@ -65,7 +65,7 @@ EXPORTS = {
     "cumayo_serve_clock_ns": ([], ctypes.c_int64),
     "cumayo_serve_run_poisson": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_uint64, _VP, _VP, _VP, _VP, _VP, _SZ,
-                                  ctypes.POINTER(ServeReport)], ctypes.c_int),
+                                  ctypes.POINTER(ServeReport), _VP, _SZ], ctypes.c_int),
 }
 
 
@@ -176,8 +176,13 @@ class Server:
 
     def run_poisson(self, rate: float, duration: float, verify_fraction: float, seed: int,
                     flat: np.ndarray, offs: np.ndarray, lens: np.ndarray, keys: np.ndarray,
-                    sigs: np.ndarray | None = None) -> dict:
+                    sigs: np.ndarray | None = None, latencies: bool = False) -> dict:
+        """Open-loop Poisson load (native generator thread: the GIL is
+        released for the whole run). With latencies=True the report carries
+        the raw latencies in ns under "latencies_ns"."""
         rep = ServeReport()
+        cap = int(rate * duration * 1.5) + 1024 if latencies else 0
+        lat = np.zeros(cap, np.int64)
         flat = np.ascontiguousarray(flat, np.uint8)
         offs = np.ascontiguousarray(offs, np.int64)
         lens = np.ascontiguousarray(lens, np.int32)
@@ -188,7 +193,11 @@ class Server:
             sp = sigs.ctypes.data
         err = self.lib.cumayo_serve_run_poisson(
             self.handle, rate, duration, verify_fraction, seed, flat.ctypes.data, offs.ctypes.data,
-            lens.ctypes.data, keys.ctypes.data, sp, int(keys.size), ctypes.byref(rep))
+            lens.ctypes.data, keys.ctypes.data, sp, int(keys.size), ctypes.byref(rep),
+            lat.ctypes.data if cap else None, cap)
         if err:
             raise RuntimeError(f"cumayo_serve_run_poisson: HIP error {err}")
-        return rep.as_dict()
+        out = rep.as_dict()
+        if latencies:
+            out["latencies_ns"] = lat[:min(cap, int(rep.completed))].copy()
+        return out

@@ -262,11 +262,14 @@ int64_t cumayo_serve_clock_ns(void);
  * of prepared requests (message i of the pool, key msg_key[i], and for
  * verification the signature sig_pool + i * sig_bytes); a fraction
  * verify_fraction of arrivals are verifications. Drains, then reports
- * latency percentiles (arrival to result). Returns 0 or an error. */
+ * latency percentiles (arrival to result); with lat_out, the first lat_cap
+ * latencies (ns, completion order) are also written there. Returns 0 or an
+ * error. */
 int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
                              double verify_fraction, uint64_t seed, const uint8_t *msg_pool,
                              const int64_t *msg_off, const int32_t *msg_len, const int32_t *msg_key,
-                             const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out);
+                             const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out,
+                             int64_t *lat_out, size_t lat_cap);
 
 #ifdef __cplusplus
 }

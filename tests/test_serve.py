@@ -137,3 +137,32 @@ def test_serve_poisson_load(engine_factory):
     assert rep["refused"] == 0
     assert rep["completed"] == rep["submitted"] > 50_000
     assert 0 < rep["p50_ms"] <= rep["p99_ms"] < 100
+
+
+def test_mixed_parameter_sets(engine_factory):
+    """MAYO_1 and MAYO_2 requests interleaved through one MixedServer give
+    each set's batch results."""
+    from cumayo_amd.mixed import MixedServer
+    engines, tables, reqs, expect = {}, {}, [], {}
+    tag = 0
+    for v in (1, 2):
+        eng = engine_factory(v)
+        esk, epk = eng.keypair_tables(3, rand_index=40 + v)
+        engines[v], tables[v] = eng, (esk, epk)
+        r, batch_sigs, exp_valid = _requests(eng, esk, epk, 90, 3, seed=50 + v)
+        for (i, op, key, msg, sig) in r:
+            reqs.append((tag, v, op, key, msg, sig))
+            expect[tag] = (v, op, batch_sigs[i].tobytes(), exp_valid.get(i, 0))
+            tag += 1
+    rng = np.random.default_rng(3)
+    order = rng.permutation(len(reqs))
+    with MixedServer(engines, tables, max_batch=32, n_slots=3) as ms:
+        assert ms.submit([reqs[k] for k in order]) == len(reqs)
+        got = ms.wait({v: [t for (t, vv, *_rest) in reqs if vv == v] for v in (1, 2)})
+    for t, (v, op, sig, bad) in expect.items():
+        gv, status, _, gsig, res = got[t]
+        assert gv == v
+        if op == 0:
+            assert status == 0 and gsig.tobytes() == sig, f"sign tag {t}"
+        else:
+            assert status == bad and res == bad, f"verify tag {t}"
