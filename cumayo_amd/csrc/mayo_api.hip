@@ -363,9 +363,10 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
         } else {
             // attempt 0 in two kernels tuned for their register needs, then
             // the rank failures (about 1 in 4,000) through the full loop
+            const int fail_first = (stages >> 8) & 0xFF;
             wv::k_sign_prep<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
-            wv::k_sign_solve<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
-            wv::k_sign_wave<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1);
+            wv::k_sign_solve<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n, fail_first);
+            wv::k_sign_wave<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1, fail_first);
         }
         e = hipGetLastError();
     }
@@ -556,6 +557,10 @@ void cumayo_serve_destroy(void *server) { delete static_cast<serve::Server *>(se
 
 size_t cumayo_serve_submit(void *server, const cumayo_request *reqs, size_t n) {
     return server ? static_cast<serve::Server *>(server)->submit(reqs, n) : 0;
+}
+
+int cumayo_serve_cancel(void *server, uint64_t tag) {
+    return server ? static_cast<serve::Server *>(server)->cancel(tag) : 0;
 }
 
 size_t cumayo_serve_poll(void *server, cumayo_completion *out, size_t max) {

@@ -598,7 +598,7 @@ __device__ __forceinline__ void build_columns(const LDS &S, uint32_t (&cols)[NCL
 // echelon form (sign.cu:10-145), and z on them is y'[pr] / p: the solution
 // with the free variables at zero is unique, so x equals the reference's.
 template <class LDS>
-__device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG]) {
+__device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = false) {
     const int lane = threadIdx.x;
     const uint8_t *rv = S.vr + R_OFF;
     // y' = y - A r: every A column times its r nibble, XOR-reduced over lanes
@@ -730,7 +730,9 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG]) {
             for (int g = 0; g < NG; ++g) S.yplanes[d * NG + g] = cols[Y_SET][d][g];
     }
     __syncthreads();
-    const int ok = rank == M;
+    // force_fail (test hook): report a singular system; x stays r, as when
+    // the rank check fails
+    const int ok = rank == M && !force_fail;
     for (int c = lane; c < KO; c += 64) {
         uint32_t xv = mvec::nib(rv, c);
         const int pr = S.prow[c];
@@ -782,8 +784,10 @@ __device__ __forceinline__ void wave_write_sig(const LDS &S, const SignItem &it)
 
 // The whole signing loop in one wave (sign.cu:244-478). With retry_only set
 // it handles only items the solve kernel left pending, from attempt 1 on.
+// Attempts below fail_first report a singular system (test hook).
 template <class Acc, class KL = RefKeys>
-__global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int retry_only) {
+__global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int retry_only,
+                                                          int fail_first = 0) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
@@ -818,7 +822,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int
         __syncthreads();
         uint32_t cols[NCL][4][NG];
         build_columns(S, cols);
-        ok = solve(S, cols);
+        ok = solve(S, cols, ctr < fail_first);
         if (ok || ctr == MAX_ATTEMPTS - 1) break;
         derive_vr(ctr + 1);
     }
@@ -852,7 +856,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
 // Batch form, attempt 0, part 2: columns of [A | y - A r], Gauss-Jordan,
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
 template <class Acc>
-__global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n) {
+__global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0) {
     __shared__ SolveShared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
@@ -864,7 +868,7 @@ __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n)
     __syncthreads();
     uint32_t cols[NCL][4][NG];
     build_columns(S, cols);
-    const int ok = solve(S, cols);
+    const int ok = solve(S, cols, fail_first > 0);
     if (!ok) {
         if (lane == 0) it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
         return;

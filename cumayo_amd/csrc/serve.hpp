@@ -156,6 +156,19 @@ class Server {
         return accepted;
     }
 
+    // Withdraws a request that is still queued; it completes with status 3.
+    int cancel(uint64_t tag) {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = pending_.begin(); it != pending_.end(); ++it)
+            if (it->tag == tag) {
+                completed_.push_back({tag, now_ns() - it->t_arrival_ns, 3});
+                pending_.erase(it);
+                ++cancelled_;
+                return 1;
+            }
+        return 0;
+    }
+
     size_t poll(cumayo_completion *out, size_t max) {
         std::lock_guard<std::mutex> lk(mu_);
         const size_t k = std::min(max, completed_.size());
@@ -172,6 +185,7 @@ class Server {
         out->max_in_flight = max_in_flight_;
         out->overlapped_launches = overlapped_;
         out->device_bytes = device_bytes_;
+        out->cancelled = cancelled_;
     }
 
     int error() const { return err_.load(); }
@@ -317,7 +331,8 @@ class Server {
     std::thread thread_;
     bool stop_ = false;
     std::atomic<int> err_{0};
-    uint64_t batches_ = 0, requests_done_ = 0, rejected_ = 0, max_in_flight_ = 0, overlapped_ = 0;
+    uint64_t batches_ = 0, requests_done_ = 0, rejected_ = 0, max_in_flight_ = 0, overlapped_ = 0,
+             cancelled_ = 0;
     size_t in_flight_requests_ = 0, device_bytes_ = 0;
 };
 

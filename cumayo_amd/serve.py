@@ -41,7 +41,8 @@ class Completion(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_uint64), ("requests", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("max_in_flight", ctypes.c_uint64),
-                ("overlapped_launches", ctypes.c_uint64), ("device_bytes", _SZ)]
+                ("overlapped_launches", ctypes.c_uint64), ("device_bytes", _SZ),
+                ("cancelled", ctypes.c_uint64)]
 
 
 class ServeReport(ctypes.Structure):
@@ -63,6 +64,7 @@ EXPORTS = {
     "cumayo_serve_poll": ([_VP, ctypes.POINTER(Completion), _SZ], _SZ),
     "cumayo_serve_get_stats": ([_VP, ctypes.POINTER(ServeStats)], None),
     "cumayo_serve_clock_ns": ([], ctypes.c_int64),
+    "cumayo_serve_cancel": ([_VP, ctypes.c_uint64], ctypes.c_int),
     "cumayo_serve_run_poisson": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_uint64, _VP, _VP, _VP, _VP, _VP, _SZ,
                                   ctypes.POINTER(ServeReport), _VP, _SZ], ctypes.c_int),
@@ -150,6 +152,10 @@ class Server:
         for r in arr[acc:]:
             self._live.pop(r.tag, None)
         return acc
+
+    def cancel(self, tag: int) -> bool:
+        """Withdraws a queued request; it then completes with status 3."""
+        return bool(self.lib.cumayo_serve_cancel(self.handle, tag))
 
     def poll(self, max_n: int = 65536):
         buf = (Completion * max_n)()

@@ -130,7 +130,11 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
  * y, then the solve, then a retry pass over the items whose first attempt
  * met a singular system). Stage 2 reads what stage 1 left in the workspace;
  * 3 runs both. Adding 4 runs stage 2 with the workgroup-per-item kernels
- * instead (same results; kept as the comparison form). */
+ * instead (same results; kept as the comparison form). Test hook: a nonzero
+ * value f in bits 8..15 makes attempts 0 .. f-1 of every item report a
+ * singular system (the retry path; f = 255 exhausts every attempt and
+ * writes the last one's vector, attempts byte 0), as the oracle's
+ * fail_first does; not with bit 4. */
 int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
                              const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
                              uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
@@ -236,12 +240,14 @@ typedef struct {
     int64_t latency_ns;     /* arrival to result on the host */
     uint8_t status;         /* verify: 0 valid / 1 invalid; sign: 0 ok, 1 all attempts failed;
                                2 = refused (key out of range, no key table for the op,
-                               or a message longer than slot_msg_bytes) */
+                               or a message longer than slot_msg_bytes);
+                               3 = cancelled before it was launched */
 } cumayo_completion;
 
 typedef struct {
     uint64_t batches, requests, rejected, max_in_flight, overlapped_launches;
     size_t device_bytes;
+    uint64_t cancelled;
 } cumayo_serve_stats;
 
 typedef struct {
@@ -256,6 +262,10 @@ void cumayo_serve_destroy(void *server);
 /* Returns the number of requests accepted (admission stops at max_queue). */
 size_t cumayo_serve_submit(void *server, const cumayo_request *reqs, size_t n);
 size_t cumayo_serve_poll(void *server, cumayo_completion *out, size_t max);
+/* Withdraws a queued request (not yet in a batch): it completes with status
+ * 3 and its buffers are no longer touched. Returns 1 if it was withdrawn, 0
+ * if it had already been launched or completed. */
+int cumayo_serve_cancel(void *server, uint64_t tag);
 void cumayo_serve_get_stats(void *server, cumayo_serve_stats *out);
 int64_t cumayo_serve_clock_ns(void);
 /* Open-loop load: Poisson arrivals at rate_per_s for duration_s over a pool

@@ -166,3 +166,29 @@ def test_mixed_parameter_sets(engine_factory):
             assert status == 0 and gsig.tobytes() == sig, f"sign tag {t}"
         else:
             assert status == bad and res == bad, f"verify tag {t}"
+
+
+def test_serve_backpressure_and_cancel(engine_factory):
+    """A full queue refuses further requests (counted as rejected); queued
+    requests can be withdrawn and complete with status 3, the others with
+    the batch results."""
+    from cumayo_amd.serve import Server
+    eng = engine_factory(1)
+    esk, epk = eng.keypair_tables(2, rand_index=8)
+    reqs, batch_sigs, _ = _requests(eng, esk, epk, 400, 2, seed=77)
+    signs = [r for r in reqs if r[1] == 0][:200]
+    with Server(eng, esk, epk, max_batch=4, n_slots=1, max_queue=60) as srv:
+        acc = srv.submit(signs)
+        assert 60 <= acc < len(signs)
+        assert srv.stats()["rejected"] == len(signs) - acc
+        accepted = [r[0] for r in signs[:acc]]
+        withdrawn = [tag for tag in accepted[-30:] if srv.cancel(tag)]
+        assert len(withdrawn) > 0
+        got = srv.wait(accepted)
+        assert srv.stats()["cancelled"] == len(withdrawn)
+    for tag in accepted:
+        status, _, sig, _ = got[tag]
+        if tag in withdrawn:
+            assert status == 3
+        else:
+            assert status == 0 and sig.tobytes() == batch_sigs[tag].tobytes()
