@@ -136,7 +136,7 @@ void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) 
     note_launch();
     k_aes_expand<<<(unsigned)n, WG, 0, stream>>>(pool, pitch, n, p1_off, p1_off);
     note_launch();
-    k_expand_L<<<(unsigned)n, WG, 0, stream>>>(pool, pitch, n);
+    wv::k_expand_L_wave<<<(unsigned)n, 64, 0, stream>>>(pool, pitch, n);
     note_launch();
 }
 
@@ -209,7 +209,7 @@ void crypto_sign_keypair(uint8_t *pool, size_t pitch, size_t batch_size, hipStre
     note_launch();
     k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
     note_launch();
-    k_keypair_P3<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
+    wv::k_keypair_P3_wave<<<(unsigned)batch_size, 64, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
     note_launch();
 }
 

@@ -231,15 +231,17 @@ __constant__ uint32_t c_aes_S[256];
 // key and out may overlap: the key is staged in LDS before any store.
 __global__ void __launch_bounds__(WG) k_aes_expand(uint8_t *pool, size_t pitch, size_t n,
                                                    size_t key_off, size_t out_off) {
-    __shared__ aes::Tables tb;
+    __shared__ aes::Replicated tb;
+    __shared__ uint32_t S[256];
     __shared__ uint32_t rk[44];
     __shared__ uint8_t key[16];
     const int tid = threadIdx.x;
     uint8_t *row = pool + (size_t)blockIdx.x * pitch;
-    for (int e = tid; e < 256; e += WG) { tb.T0[e] = c_aes_T0[e]; tb.S[e] = c_aes_S[e]; }
+    for (int e = tid; e < 256 * 32; e += WG) tb.T0[e] = c_aes_T0[e >> 5];
+    for (int e = tid; e < 256; e += WG) S[e] = c_aes_S[e];
     if (tid < 16) key[tid] = row[key_off + tid];
     __syncthreads();
-    if (tid == 0) aes::expand_key(key, rk, tb.S);
+    if (tid == 0) aes::expand_key(key, rk, S);
     __syncthreads();
     uint8_t *out = row + out_off;
     const bool aligned = (((uintptr_t)out) & 3) == 0;
@@ -256,104 +258,6 @@ __global__ void __launch_bounds__(WG) k_aes_expand(uint8_t *pool, size_t pitch, 
             if (nb > 16) nb = 16;
             for (int b = 0; b < nb; ++b) p[b] = (uint8_t)(o[b >> 2] >> (24 - 8 * (b & 3)));
         }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Key-side linear algebra
-// ---------------------------------------------------------------------------
-
-// Row r of sum_{c in cols} P1sym[r][c] * O[c][x] for all x < o, word w.
-// lower: also include c < r through P1[c][r] (the P1 + P1^T of L).
-template <bool LOWER>
-__device__ __forceinline__ void p1_times_O_row(const uint8_t *P1, const uint8_t *Onib, int r, int w,
-                                               uint32_t (&acc)[O]) {
-    const int nv = mvec::word_valid(w);
-    if (LOWER) {
-        for (int c = 0; c < r; ++c) {
-            gf16::Multiples mp(mvec::load_word(P1 + (size_t)tri_entry(c, r, V) * MB + 4 * w, nv));
-#pragma unroll
-            for (int x = 0; x < O; ++x) acc[x] ^= mp.times(Onib[c * O + x]);
-        }
-    }
-    for (int c = LOWER ? r + 1 : r; c < V; ++c) {
-        gf16::Multiples mp(mvec::load_word(P1 + (size_t)tri_entry(r, c, V) * MB + 4 * w, nv));
-#pragma unroll
-        for (int x = 0; x < O; ++x) acc[x] ^= mp.times(Onib[c * O + x]);
-    }
-}
-
-// expandsk: L = (P1 + P1^T) O + P2 in place over P2 (expandsk.cu:37-80).
-__global__ void __launch_bounds__(WG) k_expand_L(uint8_t *pool, size_t pitch, size_t n) {
-    __shared__ uint8_t Onib[V * O];
-    const int tid = threadIdx.x;
-    uint8_t *esk = pool + (size_t)blockIdx.x * pitch;
-    const uint8_t *Ob = esk + SK;
-    const uint8_t *P1 = Ob + O_BYTES;
-    uint8_t *L = (uint8_t *)P1 + P1_BYTES;
-    for (int e = tid; e < V * O; e += WG) Onib[e] = (uint8_t)mvec::nib(Ob, e);
-    __syncthreads();
-    for (int item = tid; item < MW * V; item += WG) {
-        const int w = item % MW, r = item / MW;
-        uint32_t acc[O];
-#pragma unroll
-        for (int x = 0; x < O; ++x) acc[x] = 0;
-        p1_times_O_row<true>(P1, Onib, r, w, acc);
-#pragma unroll
-        for (int x = 0; x < O; ++x) {
-            uint8_t *e = L + (size_t)(r * O + x) * MB;
-            mvec::store_mw(e, w, acc[x] ^ mvec::load_mw(e, w));
-        }
-    }
-}
-
-// keypair: P3 = Upper(O^T (P1 O + P2)) (keypair.cu:55-119). O is read from
-// the P3 area where the seed kernel parked it, then P3 overwrites it.
-__global__ void __launch_bounds__(WG) k_keypair_P3(uint8_t *pool, size_t pitch, size_t n,
-                                                   size_t pk_off, size_t p_off) {
-    __shared__ uint8_t Onib[V * O];
-    __shared__ uint32_t B[V * O * MW];
-    __shared__ uint32_t P3acc[P3_ENTRIES * MW];
-    const int tid = threadIdx.x;
-    uint8_t *row = pool + (size_t)blockIdx.x * pitch;
-    uint8_t *P3 = row + pk_off + 16;
-    const uint8_t *P1 = row + p_off;
-    const uint8_t *P2 = P1 + P1_BYTES;
-    for (int e = tid; e < V * O; e += WG) Onib[e] = (uint8_t)mvec::nib(P3, e);
-    for (int e = tid; e < P3_ENTRIES * MW; e += WG) P3acc[e] = 0;
-    __syncthreads();
-    // B[r][x] = P2[r][x] + sum_{c >= r} P1[r][c] O[c][x]
-    for (int item = tid; item < MW * V; item += WG) {
-        const int w = item % MW, r = item / MW;
-        uint32_t acc[O];
-#pragma unroll
-        for (int x = 0; x < O; ++x) acc[x] = mvec::load_mw(P2 + (size_t)(r * O + x) * MB, w);
-        p1_times_O_row<false>(P1, Onib, r, w, acc);
-#pragma unroll
-        for (int x = 0; x < O; ++x) B[(r * O + x) * MW + w] = acc[x];
-    }
-    __syncthreads();
-    // item (x, w): acc[y] = sum_r O[r][y] * B[r][x]; lands in P3 entry {x, y}
-    for (int item = tid; item < O * MW; item += WG) {
-        const int x = item / MW, w = item % MW;
-        uint32_t acc[O];
-#pragma unroll
-        for (int y = 0; y < O; ++y) acc[y] = 0;
-        for (int r = 0; r < V; ++r) {
-            gf16::Multiples mb(B[(r * O + x) * MW + w]);
-#pragma unroll
-            for (int y = 0; y < O; ++y) acc[y] ^= mb.times(Onib[r * O + y]);
-        }
-#pragma unroll
-        for (int y = 0; y < O; ++y) {
-            int e = (y <= x) ? tri_entry(y, x, O) : tri_entry(x, y, O);
-            atomicXor(&P3acc[e * MW + w], acc[y]);
-        }
-    }
-    __syncthreads();
-    for (int item = tid; item < P3_ENTRIES * MW; item += WG) {
-        const int e = item / MW, w = item % MW;
-        mvec::store_mw(P3 + (size_t)e * MB, w, P3acc[item]);
     }
 }
 

@@ -877,6 +877,170 @@ __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n,
     wave_write_sig(S, it);
 }
 
+// ---- key-side products of P1 and O (expandsk.cu:37-80, keypair.cu:55-119) ----
+// One wave per key. The entries of O are the scalars, uniform across the
+// wave, so their bits are SGPR masks; lanes are (word w, row slot) and each
+// lane accumulates the four bit planes of T_x[r] = sum_c P1'[r][c] O[c][x]
+// for its rows, as in the quadratic form. SYM: P1' = P1 + P1^T, whose
+// diagonal cancels (entry (r, c) is P1[r][c] above the diagonal, P1[c][r]
+// below). Otherwise P1' is the upper triangle with its diagonal. For the
+// rows of a chunk, columns left of the chunk read lower entries and columns
+// right of it upper entries at lane-constant offsets; only the chunk's own
+// columns need a per-lane choice. Entry words are fetched one column ahead.
+constexpr int OWD = (O + 7) / 8;            // words of one packed row of O
+constexpr int KRT = O <= 10 ? 2 : 1;        // rows per lane per chunk
+
+// ocol[c * OWD + q]: nibbles x = 8q .. 8q+7 of row c of O.
+__device__ __forceinline__ void load_ocol(const uint8_t *Onib, uint32_t *ocol) {
+    for (int c = threadIdx.x; c < V; c += 64)
+#pragma unroll
+        for (int q = 0; q < OWD; ++q) {
+            uint32_t v = 0;
+            for (int x = 8 * q; x < O && x < 8 * q + 8; ++x) v |= mvec::nib(Onib, c * O + x) << (4 * (x - 8 * q));
+            ocol[c * OWD + q] = v;
+        }
+}
+
+// row_done(r, T) is called for each valid row of the lane with T[x] = T_x[r]
+// (this lane's word w).
+template <bool SYM, class RowDone>
+__device__ __forceinline__ void p1_times_O(const uint8_t *P1, const uint32_t *ocol, RowDone &&row_done) {
+    const int lane = threadIdx.x;
+    const int w = lane % MW, slot = lane / MW;
+    const bool active = slot < SL;
+    const int nv = mvec::word_valid(w);
+    constexpr int RPC = KRT * SL;
+    constexpr int NCH = (V + RPC - 1) / RPC;
+#pragma unroll 1
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int r0 = ch * RPC, r1 = (r0 + RPC < V ? r0 + RPC : V) - 1;
+        int rr[KRT];
+        bool val[KRT];
+        uint32_t up[KRT], lo[KRT];
+        uint32_t Tb[KRT][O][4];
+#pragma unroll
+        for (int tt = 0; tt < KRT; ++tt) {
+            rr[tt] = r0 + tt * SL + slot;
+            val[tt] = active && rr[tt] < V;
+            const int r = val[tt] ? rr[tt] : 0;
+            up[tt] = (uint32_t)((tri_entry(r, r, V) - r) * MB + 4 * w);  // + c MB: entry (r, c)
+            lo[tt] = (uint32_t)(r * MB + 4 * w);                         // + (tri(c,c) - c) MB: entry (c, r)
+#pragma unroll
+            for (int x = 0; x < O; ++x)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) Tb[tt][x][b] = 0;
+        }
+        // address of column c for row slot tt (phase: 0 below, 1 mixed, 2 above)
+        auto offset = [&](int c, int tt) -> uint32_t {
+            const uint32_t lb = (uint32_t)((tri_entry(c, c, V) - c) * MB), ub = (uint32_t)(c * MB);
+            if (c < r0) return lo[tt] + lb;
+            if (c > r1) return up[tt] + ub;
+            return (SYM && c < rr[tt]) ? lo[tt] + lb : up[tt] + ub;
+        };
+        wv::EntryLoad<RefKeys> ld[KRT];
+        const int cstart = SYM ? 0 : r0;
+#pragma unroll
+        for (int tt = 0; tt < KRT; ++tt) ld[tt].fetch(P1, offset(cstart, tt), nv);
+#pragma unroll 1
+        for (int c = cstart; c < V; ++c) {
+            uint32_t x[KRT];
+#pragma unroll
+            for (int tt = 0; tt < KRT; ++tt) {
+                x[tt] = ld[tt].word(nv);
+                // the chunk's own columns: the diagonal cancels (SYM); the
+                // upper product stops below the diagonal
+                if (c >= r0 && c <= r1) x[tt] = (SYM ? c == rr[tt] : c < rr[tt]) ? 0u : x[tt];
+            }
+            if (c + 1 < V) {
+#pragma unroll
+                for (int tt = 0; tt < KRT; ++tt) ld[tt].fetch(P1, offset(c + 1, tt), nv);
+            }
+            uint32_t oc[OWD];
+#pragma unroll
+            for (int q = 0; q < OWD; ++q) oc[q] = uni(ocol[c * OWD + q]);
+#pragma unroll
+            for (int xo = 0; xo < O; ++xo)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t m = (uint32_t)(-(int32_t)((oc[xo >> 3] >> (4 * (xo & 7) + b)) & 1u));
+#pragma unroll
+                    for (int tt = 0; tt < KRT; ++tt) Tb[tt][xo][b] = gf16::axm(Tb[tt][xo][b], x[tt], m);
+                }
+        }
+#pragma unroll
+        for (int tt = 0; tt < KRT; ++tt) {
+            if (!val[tt]) continue;
+            uint32_t T[O];
+#pragma unroll
+            for (int xo = 0; xo < O; ++xo)
+                T[xo] = gf16::mul_a(gf16::mul_a(gf16::mul_a(Tb[tt][xo][3]) ^ Tb[tt][xo][2]) ^ Tb[tt][xo][1]) ^
+                        Tb[tt][xo][0];
+            row_done(rr[tt], T);
+        }
+    }
+}
+
+// expandsk: L = (P1 + P1^T) O + P2 in place over P2.
+__global__ void __launch_bounds__(64, 3) k_expand_L_wave(uint8_t *pool, size_t pitch, size_t n) {
+    __shared__ uint32_t ocol[V * OWD];
+    uint8_t *esk = pool + (size_t)blockIdx.x * pitch;
+    const uint8_t *P1 = esk + SK + O_BYTES;
+    uint8_t *L = (uint8_t *)P1 + P1_BYTES;
+    load_ocol(esk + SK, ocol);
+    __syncthreads();
+    const int w = threadIdx.x % MW;
+    p1_times_O<true>(P1, ocol, [&](int r, const uint32_t (&T)[O]) {
+#pragma unroll
+        for (int xo = 0; xo < O; ++xo) {
+            uint8_t *e = L + (size_t)(r * O + xo) * MB;
+            mvec::store_mw(e, w, T[xo] ^ mvec::load_mw(e, w));
+        }
+    });
+}
+
+// keypair: P3 = Upper(O^T (P1 O + P2)). O is read from the P3 area where the
+// seed kernel parked it, then P3 overwrites it. Each row r of B = P1 O + P2
+// adds O[r][x] B[r][y] + O[r][y] B[r][x] into entry {x, y} (x < y) and
+// O[r][x] B[r][x] into {x, x}, summed over the wave in LDS.
+__global__ void __launch_bounds__(64, 2) k_keypair_P3_wave(uint8_t *pool, size_t pitch, size_t n,
+                                                           size_t pk_off, size_t p_off) {
+    __shared__ uint32_t ocol[V * OWD];
+    __shared__ uint32_t P3acc[P3_ENTRIES * MW];
+    uint8_t *row = pool + (size_t)blockIdx.x * pitch;
+    uint8_t *P3 = row + pk_off + 16;
+    const uint8_t *P1 = row + p_off;
+    const uint8_t *P2 = P1 + P1_BYTES;
+    load_ocol(P3, ocol);
+    for (int e = threadIdx.x; e < P3_ENTRIES * MW; e += 64) P3acc[e] = 0;
+    __syncthreads();
+    const int w = threadIdx.x % MW;
+    p1_times_O<false>(P1, ocol, [&](int r, const uint32_t (&T)[O]) {
+        uint32_t om[O][4];  // bit masks of O[r][x]
+#pragma unroll
+        for (int x = 0; x < O; ++x) {
+            const uint32_t s = (ocol[r * OWD + (x >> 3)] >> (4 * (x & 7))) & 0xFu;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) om[x][b] = gf16::bitmask(s, b);
+        }
+#pragma unroll
+        for (int y = 0; y < O; ++y) {
+            const gf16::Multiples mb(T[y] ^ mvec::load_mw(P2 + (size_t)(r * O + y) * MB, w));  // B[r][y]
+#pragma unroll
+            for (int x = 0; x < O; ++x) {
+                // O[r][x] B[r][y] lands in {min(x,y), max(x,y)}
+                const uint32_t v = gf16::mac(0u, mb.x0, mb.x1, mb.x2, mb.x3, om[x][0], om[x][1], om[x][2], om[x][3]);
+                const int e = x <= y ? tri_entry(x, y, O) : tri_entry(y, x, O);
+                atomicXor(&P3acc[e * MW + w], v);
+            }
+        }
+    });
+    __syncthreads();
+    for (int item = threadIdx.x; item < P3_ENTRIES * MW; item += 64) {
+        const int e = item / MW, ww = item % MW;
+        mvec::store_mw(P3 + (size_t)e * MB, ww, P3acc[item]);
+    }
+}
+
 // Verification, one wave per item (verify.cu:9-211).
 template <class Acc, class KL = RefKeys>
 __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
