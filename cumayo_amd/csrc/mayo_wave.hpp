@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n,
 // right of it upper entries at lane-constant offsets; only the chunk's own
 // columns need a per-lane choice. Entry words are fetched one column ahead.
 constexpr int OWD = (O + 7) / 8;            // words of one packed row of O
-constexpr int KRT = O <= 10 ? 2 : 1;        // rows per lane per chunk
+constexpr int KRT = O <= 12 ? 2 : 1;        // rows per lane per chunk
 
 // ocol[c * OWD + q]: nibbles x = 8q .. 8q+7 of row c of O.
 __device__ __forceinline__ void load_ocol(const uint8_t *Onib, uint32_t *ocol) {
@@ -937,35 +937,46 @@ __device__ __forceinline__ void p1_times_O(const uint8_t *P1, const uint32_t *oc
             if (c > r1) return up[tt] + ub;
             return (SYM && c < rr[tt]) ? lo[tt] + lb : up[tt] + ub;
         };
-        wv::EntryLoad<RefKeys> ld[KRT];
+        // the loads of the next PF columns are in flight during each column
+        constexpr int PF = 1;
+        wv::EntryLoad<RefKeys> ld[PF][KRT];
         const int cstart = SYM ? 0 : r0;
 #pragma unroll
-        for (int tt = 0; tt < KRT; ++tt) ld[tt].fetch(P1, offset(cstart, tt), nv);
+        for (int k = 0; k < PF; ++k)
+            if (cstart + k < V) {
+#pragma unroll
+                for (int tt = 0; tt < KRT; ++tt) ld[k][tt].fetch(P1, offset(cstart + k, tt), nv);
+            }
 #pragma unroll 1
-        for (int c = cstart; c < V; ++c) {
-            uint32_t x[KRT];
+        for (int c0 = cstart; c0 < V; c0 += PF) {
 #pragma unroll
-            for (int tt = 0; tt < KRT; ++tt) {
-                x[tt] = ld[tt].word(nv);
-                // the chunk's own columns: the diagonal cancels (SYM); the
-                // upper product stops below the diagonal
-                if (c >= r0 && c <= r1) x[tt] = (SYM ? c == rr[tt] : c < rr[tt]) ? 0u : x[tt];
-            }
-            if (c + 1 < V) {
+            for (int k = 0; k < PF; ++k) {
+                const int c = c0 + k;
+                if (c >= V) break;
+                uint32_t x[KRT];
 #pragma unroll
-                for (int tt = 0; tt < KRT; ++tt) ld[tt].fetch(P1, offset(c + 1, tt), nv);
-            }
-            uint32_t oc[OWD];
-#pragma unroll
-            for (int q = 0; q < OWD; ++q) oc[q] = uni(ocol[c * OWD + q]);
-#pragma unroll
-            for (int xo = 0; xo < O; ++xo)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const uint32_t m = (uint32_t)(-(int32_t)((oc[xo >> 3] >> (4 * (xo & 7) + b)) & 1u));
-#pragma unroll
-                    for (int tt = 0; tt < KRT; ++tt) Tb[tt][xo][b] = gf16::axm(Tb[tt][xo][b], x[tt], m);
+                for (int tt = 0; tt < KRT; ++tt) {
+                    x[tt] = ld[k][tt].word(nv);
+                    // the chunk's own columns: the diagonal cancels (SYM); the
+                    // upper product stops below the diagonal
+                    if (c >= r0 && c <= r1) x[tt] = (SYM ? c == rr[tt] : c < rr[tt]) ? 0u : x[tt];
                 }
+                if (c + PF < V) {
+#pragma unroll
+                    for (int tt = 0; tt < KRT; ++tt) ld[k][tt].fetch(P1, offset(c + PF, tt), nv);
+                }
+                uint32_t oc[OWD];
+#pragma unroll
+                for (int q = 0; q < OWD; ++q) oc[q] = uni(ocol[c * OWD + q]);
+#pragma unroll
+                for (int xo = 0; xo < O; ++xo)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const uint32_t m = (uint32_t)(-(int32_t)((oc[xo >> 3] >> (4 * (xo & 7) + b)) & 1u));
+#pragma unroll
+                        for (int tt = 0; tt < KRT; ++tt) Tb[tt][xo][b] = gf16::axm(Tb[tt][xo][b], x[tt], m);
+                    }
+            }
         }
 #pragma unroll
         for (int tt = 0; tt < KRT; ++tt) {
