@@ -33,16 +33,18 @@ constexpr int NCL = (NCOL + 63) / 64;          // columns held per lane
 constexpr int OVW = (PAIRS + 7) / 8 + 1;       // overflow words of an unreduced sum
 constexpr int UW = MW + OVW;                   // words of an unreduced sum
 // quad-form rows per lane per chunk, and the waves per SIMD the quad-form
-// kernels are compiled for (the pair accumulators and QF_RT x 4K bit-plane
-// accumulators must fit the VGPR budget 512 / QF_OCC)
+// kernels are compiled for (VGPR budget 512 / QF_OCC for the pair
+// accumulators and QF_RT x 4K bit-plane accumulators). Two rows and three
+// waves were the fastest setting measured for every parameter set; with one
+// row the 4K scalar masks per column make the SALU the bottleneck for k >= 11.
 #ifndef MAYO_QF_RT
 #define MAYO_QF_RT 0
 #endif
 #ifndef MAYO_QF_OCC
 #define MAYO_QF_OCC 0
 #endif
-constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : (K <= 4 ? 2 : K <= 10 ? 2 : 1);
-constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : K <= 11 ? 3 : 2);
+constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : 2;
+constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : 3);
 // waves per SIMD of the solve kernel (the bitsliced columns of [A | y'])
 constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
 // column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
