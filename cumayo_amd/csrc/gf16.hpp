@@ -65,4 +65,51 @@ __device__ __forceinline__ uint32_t inv(uint32_t a) {
 // Replicate a nibble into all 8 positions of a word.
 __device__ __forceinline__ uint32_t splat(uint32_t s) { return (s & 0xF) * 0x11111111u; }
 
+// ---- planar words ----
+// The same 8 elements with their bits grouped by plane: bit d of element e
+// at bit 8d + e, so byte d is bit plane d. Sums and masked sums are the same
+// bitwise operations in either form, but multiplication by a is a byte
+// rotation plus one XOR: planes (P0, P1, P2, P3) -> (P3, P0 + P3, P1, P2).
+namespace planar {
+
+// exchange the bits selected by m with the bits D positions above them
+template <int D>
+__device__ __forceinline__ uint32_t delta_swap(uint32_t x, uint32_t m) {
+    const uint32_t t = (x ^ (x >> D)) & m;
+    return x ^ t ^ (t << D);
+}
+
+// nibble-packed -> planar: bit 4e + d -> bit 8d + e, four index-bit swaps
+__device__ __forceinline__ uint32_t from_nibbles(uint32_t x) {
+    x = delta_swap<14>(x, 0x0000CCCCu);
+    x = delta_swap<7>(x, 0x00AA00AAu);
+    x = delta_swap<2>(x, 0x0C0C0C0Cu);
+    return delta_swap<1>(x, 0x22222222u);
+}
+
+// planar -> nibble-packed (the same swaps in reverse order)
+__device__ __forceinline__ uint32_t to_nibbles(uint32_t x) {
+    x = delta_swap<1>(x, 0x22222222u);
+    x = delta_swap<2>(x, 0x0C0C0C0Cu);
+    x = delta_swap<7>(x, 0x00AA00AAu);
+    return delta_swap<14>(x, 0x0000CCCCu);
+}
+
+__device__ __forceinline__ uint32_t mul_a(uint32_t w) {
+    return __builtin_amdgcn_alignbit(w, w, 24) ^ ((w >> 16) & 0xFF00u);
+}
+
+// a * acc + t (one Horner step)
+__device__ __forceinline__ uint32_t mul_a_add(uint32_t acc, uint32_t t) {
+    return __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_alignbit(acc, acc, 24), (acc >> 16) & 0xFF00u, t, 0x96);
+}
+
+struct Multiples {
+    uint32_t x0, x1, x2, x3;
+    __device__ __forceinline__ explicit Multiples(uint32_t x) {
+        x0 = x; x1 = mul_a(x0); x2 = mul_a(x1); x3 = mul_a(x2);
+    }
+};
+
+}  // namespace planar
 }  // namespace gf16

@@ -365,7 +365,7 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             // the rank failures (about 1 in 4,000) through the full loop
             const int fail_first = (stages >> 8) & 0xFF;
             wv::k_sign_prep<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
-            wv::k_sign_solve<TableSign><<<(unsigned)n, 64, 0, stream>>>(acc, n, fail_first);
+            wv::k_sign_solve<TableSign, KL::PLANAR><<<(unsigned)n, 64, 0, stream>>>(acc, n, fail_first);
             wv::k_sign_wave<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1, fail_first);
         }
         e = hipGetLastError();

@@ -43,7 +43,12 @@ constexpr int UW = MW + OVW;                   // words of an unreduced sum
 #ifndef MAYO_QF_OCC
 #define MAYO_QF_OCC 0
 #endif
+#ifndef MAYO_QF_PF
+#define MAYO_QF_PF 0
+#endif
 constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : 2;
+// key-entry columns whose loads are in flight ahead of the one being summed
+constexpr int QF_PF = MAYO_QF_PF ? MAYO_QF_PF : 1;
 constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : 3);
 // waves per SIMD of the solve kernel (the bitsliced columns of [A | y'])
 constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
@@ -106,16 +111,20 @@ struct Shared {
 // Reference rows (esk = sk || O || P1 || L, epk = P1 || P2 || P3): entries
 // MB bytes apart at any alignment. Prepared rows: the same blocks with every
 // entry padded with zero bytes to MBP (a multiple of 4) and each block
-// 16-byte aligned, so that a word of an entry is one aligned dword load.
+// 16-byte aligned, so that a word of an entry is one aligned dword load, and
+// every entry word stored planar (gf16::planar), which makes the per-row
+// multiplications by a of the quadratic form a third as costly.
 constexpr int MBP = (MB + 3) & ~3;
 struct RefKeys {
     static constexpr bool ALIGNED = false;
+    static constexpr bool PLANAR = false;
     static constexpr int EB = MB;
     static constexpr uint32_t ESK_P1 = SK + O_BYTES, ESK_L = ESK_P1 + P1_BYTES;
     static constexpr uint32_t EPK_P1 = 0, EPK_P2 = P1_BYTES, EPK_P3 = P1_BYTES + P2_BYTES;
 };
 struct PrepKeys {
     static constexpr bool ALIGNED = true;
+    static constexpr bool PLANAR = true;
     static constexpr int EB = MBP;
     static constexpr uint32_t ESK_P1 = (SK + O_BYTES + 15) / 16 * 16;
     static constexpr uint32_t ESK_L = ESK_P1 + P1_ENTRIES * MBP;
@@ -128,8 +137,8 @@ struct PrepKeys {
 // Rewrite reference key rows in the prepared layout. Each destination dword
 // is assembled from the source bytes it maps to: the header bytes [0, hdr)
 // are copied, region r maps entry e (MB bytes at src + e MB) to dst + e MBP
-// with zero padding, everything else is zero. Grid: (dwords per row / 256
-// rounded up) x keys.
+// with zero padding and each of its dwords made planar, everything else is
+// zero. Grid: (dwords per row / 256 rounded up) x keys.
 struct PrepRegion {
     uint32_t src, dst, count;
 };
@@ -140,6 +149,12 @@ __global__ void __launch_bounds__(256) k_prepare_keys(const uint8_t *__restrict_
     const uint32_t o = 4 * (blockIdx.x * 256 + threadIdx.x);
     if (o >= dst_bytes) return;
     const uint8_t *s = src + (size_t)blockIdx.y * src_stride;
+    // region starts are 16-byte aligned and MBP a multiple of 4: a dword
+    // lies in one entry of one region or in none
+    const PrepRegion rg[3] = {r0, r1, r2};
+    bool in_region = false;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) in_region |= o >= rg[r].dst && o < rg[r].dst + rg[r].count * (uint32_t)MBP;
     uint32_t v = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -148,7 +163,6 @@ __global__ void __launch_bounds__(256) k_prepare_keys(const uint8_t *__restrict_
         if (ob < hdr) {
             byte = s[ob];
         } else {
-            const PrepRegion rg[3] = {r0, r1, r2};
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 if (ob >= rg[r].dst && ob < rg[r].dst + rg[r].count * (uint32_t)MBP) {
@@ -159,6 +173,7 @@ __global__ void __launch_bounds__(256) k_prepare_keys(const uint8_t *__restrict_
         }
         v |= byte << (8 * i);
     }
+    if (in_region) v = gf16::planar::from_nibbles(v);
     *(uint32_t *)(dst + (size_t)blockIdx.y * dst_stride + o) = v;
 }
 
@@ -202,7 +217,9 @@ struct EntryLoad {
 // (verification) the rows are those of [[P1, P2], [0, P3]] over c < n
 // (verify.cu:72-182). Each P row then adds S_a[r] T_j[r] into the pair
 // accumulators U[pair(a, j)] (ordered (a, j) both ways), XOR-reduced into
-// Ulds at the end; each L row stores M_j[x] into Mv.
+// Ulds at the end; each L row stores M_j[x] into Mv. Words keep the key's
+// form (planar for KL::PLANAR) throughout; Ulds is returned nibble-packed,
+// Mv in the key's form.
 //
 // Lanes = (word w, row slot); a chunk gives each slot QF_RT rows, rows in
 // ascending start column so that every row of a chunk starts within a few
@@ -257,42 +274,52 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
 #pragma unroll
                 for (int b = 0; b < 4; ++b) Tb[tt][j][b] = 0;
         }
-        // one column: every row of the chunk takes its entry at c (fetched
-        // during the previous column); the next column's loads are issued
-        // before the accumulation when the segment has one
-        EntryLoad<KL> ld[QF_RT];
-        auto fetch = [&]() {
+        // one column: every row of the chunk takes its entry at c, fetched
+        // QF_PF columns earlier into ring slot k (columns c0 + k, c0 + k +
+        // QF_PF, ... of a segment); off[tt] is the next column to fetch
+        EntryLoad<KL> ld[QF_PF][QF_RT];
+        auto fetch = [&](int k) {
 #pragma unroll
-            for (int tt = 0; tt < QF_RT; ++tt) ld[tt].fetch(key, off[tt], nv);
+            for (int tt = 0; tt < QF_RT; ++tt) {
+                ld[k][tt].fetch(key, off[tt], nv);
+                off[tt] += stride[tt];
+            }
         };
-        auto column = [&](int c, bool masked, bool more) {
+        auto column = [&](int c, int k, bool masked, bool more) {
             const uint64_t sc = uni64(scol[c]);
+            // masks from the 32-bit halves: one scalar bit-field extract each
+            const uint32_t sh[2] = {(uint32_t)sc, (uint32_t)(sc >> 32)};
             uint32_t x[QF_RT];
 #pragma unroll
             for (int tt = 0; tt < QF_RT; ++tt) {
-                x[tt] = ld[tt].word(nv);
+                x[tt] = ld[k][tt].word(nv);
                 if (masked) x[tt] = c >= st[tt] ? x[tt] : 0u;
-                off[tt] += stride[tt];
             }
-            if (more) fetch();
+            if (more) fetch(k);
 #pragma unroll
             for (int j = 0; j < K; ++j)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const uint32_t m = (uint32_t)(-(int32_t)((sc >> (4 * j + b)) & 1u));
+                    const int bit = 4 * j + b;
+                    const uint32_t m = gf16::bitmask(sh[bit >> 5], bit & 31);
 #pragma unroll
                     for (int tt = 0; tt < QF_RT; ++tt) Tb[tt][j][b] = gf16::axm(Tb[tt][j][b], x[tt], m);
                 }
         };
+        // columns [c0, cend), masked below cm
+        auto segment = [&](int c0, int cm, int cend) {
+#pragma unroll
+            for (int k = 0; k < QF_PF; ++k)
+                if (c0 + k < cend) fetch(k);
+#pragma unroll 1
+            for (int c = c0; c < cend; c += QF_PF) {
+#pragma unroll
+                for (int k = 0; k < QF_PF; ++k)
+                    if (c + k < cend) column(c + k, k, c + k < cm, c + k + QF_PF < cend);
+            }
+        };
         // columns < V (P1 and L entries)
-        if (c_lo < V) {
-            const int cm = c_free < V ? c_free : V;
-            fetch();
-#pragma unroll 1
-            for (int c = c_lo; c < cm; ++c) column(c, true, c + 1 < V);
-#pragma unroll 1
-            for (int c = cm; c < V; ++c) column(c, false, c + 1 < V);
-        }
+        if (c_lo < V) segment(c_lo, c_free < V ? c_free : V, V);
         if constexpr (FULL) {
             // columns >= V: P2 for rows < V, P3 for rows >= V
             const int cb = c_lo > V ? c_lo : V;
@@ -310,12 +337,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                     stride[tt] = EB;
                 }
             }
-            const int cm = c_free > cb ? c_free : cb;
-            fetch();
-#pragma unroll 1
-            for (int c = cb; c < cm; ++c) column(c, true, c + 1 < N);
-#pragma unroll 1
-            for (int c = cm; c < N; ++c) column(c, false, c + 1 < N);
+            segment(cb, c_free > cb ? c_free : cb, N);
         }
         // rows done: T_j = sum_b a^b T_jb
 #pragma unroll
@@ -323,9 +345,15 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
             const int vr = v0 + tt * SL + slot;
             uint32_t T[K];
 #pragma unroll
-            for (int j = 0; j < K; ++j)
-                T[j] = gf16::mul_a(gf16::mul_a(gf16::mul_a(Tb[tt][j][3]) ^ Tb[tt][j][2]) ^ Tb[tt][j][1]) ^
-                       Tb[tt][j][0];
+            for (int j = 0; j < K; ++j) {
+                if constexpr (KL::PLANAR) {
+                    using namespace gf16::planar;
+                    T[j] = mul_a_add(mul_a_add(mul_a_add(Tb[tt][j][3], Tb[tt][j][2]), Tb[tt][j][1]), Tb[tt][j][0]);
+                } else {
+                    T[j] = gf16::mul_a(gf16::mul_a(gf16::mul_a(Tb[tt][j][3]) ^ Tb[tt][j][2]) ^ Tb[tt][j][1]) ^
+                           Tb[tt][j][0];
+                }
+            }
             if (!active || vr >= NVR) continue;
             if (vr < NLR) {
 #pragma unroll
@@ -339,7 +367,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                     for (int b = 0; b < 4; ++b) am[a][b] = gf16::bitmask((uint32_t)(sr >> (4 * a)), b);
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    const gf16::Multiples mt(T[j]);
+                    const std::conditional_t<KL::PLANAR, gf16::planar::Multiples, gf16::Multiples> mt(T[j]);
 #pragma unroll
                     for (int a = 0; a < K; ++a) {
                         const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
@@ -353,6 +381,10 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
     if (active) {
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p) atomicXor(&Ulds[p * MW + w], U[p]);
+    }
+    if constexpr (KL::PLANAR) {
+        __syncthreads();
+        for (int e = lane; e < PAIRS * MW; e += 64) Ulds[e] = gf16::planar::to_nibbles(Ulds[e]);
     }
 }
 
@@ -463,10 +495,10 @@ struct SolveShared {
     uint8_t mul[256];
 };
 
-// Bit planes of the K*O nibble-packed vectors M_j[x] (MW words each, vector
-// v = j*O + x at src + v*MW, global or LDS): Mb[(v*4 + d)*NG + g] holds bit d
-// of rows 32g .. 32g+31.
-template <class SRC>
+// Bit planes of the K*O vectors M_j[x] (MW words each, nibble-packed or
+// planar (PL), vector v = j*O + x at src + v*MW, global or LDS):
+// Mb[(v*4 + d)*NG + g] holds bit d of rows 32g .. 32g+31.
+template <bool PL, class SRC>
 __device__ __forceinline__ void planes_of_M(const SRC *src, uint32_t *Mb) {
     for (int it = threadIdx.x; it < KO * NG; it += 64) {
         const int v = it / NG, g = it % NG;
@@ -477,7 +509,7 @@ __device__ __forceinline__ void planes_of_M(const SRC *src, uint32_t *Mb) {
         for (int d = 0; d < 4; ++d) {
             uint32_t o = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) o |= nib_plane8(w[q], d) << (8 * q);
+            for (int q = 0; q < 4; ++q) o |= (PL ? (w[q] >> (8 * d)) & 0xFFu : nib_plane8(w[q], d)) << (8 * q);
             Mb[(v * 4 + d) * NG + g] = o;
         }
     }
@@ -590,6 +622,35 @@ __device__ __forceinline__ void build_columns(const LDS &S, uint32_t (&cols)[NCL
     }
 }
 
+// GF(16) product at compile time (x^4 + x + 1)
+__host__ __device__ constexpr uint32_t gf_mul_c(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i)
+        if ((b >> i) & 1u) r ^= a << i;
+    for (int i = 6; i >= 4; --i)
+        if ((r >> i) & 1u) r ^= 0x13u << (i - 4);
+    return r;
+}
+// a^b * p^-1 as a 64-bit table over p (nibble p; 0 -> 0), from the inverse
+// table 0x834A5C2F67BDE910 (nibble a = a^-1)
+__host__ __device__ constexpr uint64_t im_table(int b) {
+    uint64_t t = 0;
+    for (int p = 1; p < 16; ++p)
+        t |= (uint64_t)gf_mul_c(1u << b, (uint32_t)(0x834A5C2F67BDE910ull >> (4 * p)) & 0xFu) << (4 * p);
+    return t;
+}
+constexpr uint64_t IM_TABLE[4] = {im_table(0), im_table(1), im_table(2), im_table(3)};
+static_assert(IM_TABLE[0] == 0x834A5C2F67BDE910ull, "inverse table");
+
+// f(integral_constant<int, g>) for the runtime word index g < NG (uniform)
+template <int G = 0, class F>
+__device__ __forceinline__ void dispatch_word(int g, F &&f) {
+    if constexpr (G < NG) {
+        if (g == G) f(std::integral_constant<int, G>{});
+        else dispatch_word<G + 1>(g, f);
+    }
+}
+
 // Gauss-Jordan on the columns of [A | y - A r]; x (S.x) = r + z. Returns 1
 // on full rank. Each pivot step: the pivot column f is broadcast (SGPRs),
 // its pivot row pr is the lowest unused row with a nonzero entry, p = f[pr];
@@ -640,8 +701,6 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
 #pragma unroll
             for (int g = 0; g < NG; ++g) cols[Y_SET][d][g] ^= S.Ar[d * NG + g];
     }
-    // GF(16) inverses as a 64-bit table, nibble a = a^-1 (0 -> 0)
-    constexpr uint64_t INV = 0x834A5C2F67BDE910ull;
     uint32_t used[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) used[g] = 0;
@@ -655,71 +714,70 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
         for (int pl = 0; pl < nl; ++pl) {
             const int pc = col_of(ps, pl);
             if (pc >= KO) break;
-            uint32_t f[4][NG];
-#pragma unroll
-            for (int d = 0; d < 4; ++d)
-#pragma unroll
-                for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
-            int pr = -1;
+            // every lane finds the pivot row its own column of set ps would
+            // have (a vector search: the scalar unit is the busier one);
+            // lane pl's is the one
+            int cand = -1;
 #pragma unroll
             for (int g = NG - 1; g >= 0; --g) {
-                const uint32_t nz = (f[0][g] | f[1][g] | f[2][g] | f[3][g]) & ~used[g];
-                if (nz) pr = 32 * g + __builtin_ctz(nz);
+                const uint32_t nz =
+                    (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~used[g];
+                cand = nz ? 32 * g + (int)__builtin_ctz(nz) : cand;
             }
+            const int pr = __builtin_amdgcn_readlane(cand, pl);
             if (pr >= 0) {
-                const int pg = pr >> 5, pb = pr & 31;
-                uint32_t p = 0;
+                uint32_t f[4][NG];
 #pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    if (g == pg) {
+                for (int d = 0; d < 4; ++d)
 #pragma unroll
-                        for (int d = 0; d < 4; ++d) {
-                            p |= ((f[d][g] >> pb) & 1u) << d;
-                            f[d][g] &= ~(1u << pb);
-                        }
-                    }
-                const uint32_t ip = (uint32_t)(INV >> (4 * p)) & 0xFu;
-                // a^b * p^-1 for the four bits of c[pr]
-                uint32_t im[4];
-                im[0] = ip;
-#pragma unroll
-                for (int b = 1; b < 4; ++b) im[b] = gf16::mul_a(im[b - 1]) & 0xFu;
-#pragma unroll
-                for (int s = 0; s < NCL; ++s) {
-                    // sets whose columns all precede pc need no update (pivot
-                    // columns are p e_pr, which the later steps leave alone;
-                    // free columns do not enter x), unless they hold y'
-                    if (s != Y_SET && set_rank(s) < ord) continue;
-                    uint32_t q = 0;
+                    for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
+                const int pb = pr & 31;
+                // specialised on the pivot row's word pg (uniform), so that
+                // no per-lane word selection is needed
+                dispatch_word(pr >> 5, [&](auto Gc) {
+                    constexpr int pg = decltype(Gc)::value;
+                    uint32_t p = 0;
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
-                        uint32_t wsel = cols[s][d][0];
-#pragma unroll
-                        for (int g = 1; g < NG; ++g) wsel = (g == pg) ? cols[s][d][g] : wsel;
-                        const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)wsel, (uint32_t)pb, 1u);
-                        q = gf16::axm(q, m, im[d]);
+                        p |= ((f[d][pg] >> pb) & 1u) << d;
+                        f[d][pg] &= ~(1u << pb);
                     }
-                    uint32_t qm[4];
+                    // a^b / p for the four bits of c[pr]
+                    const uint32_t sh = 4 * p;
+                    const uint32_t im[4] = {(uint32_t)(IM_TABLE[0] >> sh) & 0xFu, (uint32_t)(IM_TABLE[1] >> sh) & 0xFu,
+                                            (uint32_t)(IM_TABLE[2] >> sh) & 0xFu, (uint32_t)(IM_TABLE[3] >> sh) & 0xFu};
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) qm[b] = gf16::bitmask(q, b);
+                    for (int s = 0; s < NCL; ++s) {
+                        // sets whose columns all precede pc need no update (pivot
+                        // columns are p e_pr, which the later steps leave alone;
+                        // free columns do not enter x), unless they hold y'
+                        if (s != Y_SET && set_rank(s) < ord) continue;
+                        uint32_t q = 0;
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) {
-                        uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
-                        uint32_t c0 = cols[s][0][g], c1 = cols[s][1][g], c2 = cols[s][2][g], c3 = cols[s][3][g];
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            c0 = gf16::axm(c0, a0, qm[b]); c1 = gf16::axm(c1, a1, qm[b]);
-                            c2 = gf16::axm(c2, a2, qm[b]); c3 = gf16::axm(c3, a3, qm[b]);
-                            bs_mul_a(a0, a1, a2, a3);
+                        for (int d = 0; d < 4; ++d) {
+                            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)cols[s][d][pg], (uint32_t)pb, 1u);
+                            q = gf16::axm(q, m, im[d]);
                         }
-                        cols[s][0][g] = c0; cols[s][1][g] = c1; cols[s][2][g] = c2; cols[s][3][g] = c3;
-                    }
-                }
+                        uint32_t qm[4];
 #pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    if (g == pg) used[g] |= 1u << pb;
+                        for (int b = 0; b < 4; ++b) qm[b] = gf16::bitmask(q, b);
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) {
+                            uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
+                            uint32_t c0 = cols[s][0][g], c1 = cols[s][1][g], c2 = cols[s][2][g], c3 = cols[s][3][g];
+#pragma unroll
+                            for (int b = 0; b < 4; ++b) {
+                                c0 = gf16::axm(c0, a0, qm[b]); c1 = gf16::axm(c1, a1, qm[b]);
+                                c2 = gf16::axm(c2, a2, qm[b]); c3 = gf16::axm(c3, a3, qm[b]);
+                                bs_mul_a(a0, a1, a2, a3);
+                            }
+                            cols[s][0][g] = c0; cols[s][1][g] = c1; cols[s][2][g] = c2; cols[s][3][g] = c3;
+                        }
+                    }
+                    used[pg] |= 1u << pb;
+                    S.pinv[pc] = (uint8_t)im[0];
+                });
                 ++rank;
-                S.pinv[pc] = (uint8_t)ip;
             }
             S.prow[pc] = (int16_t)pr;  // same value from every lane
         }
@@ -820,7 +878,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int
         quad_form<KL, O, false>(it.esk, S.scol, S.U, S.Mv);
         __syncthreads();
         wave_y(S, S.t);
-        planes_of_M(S.Mv, S.Mb);  // Mb overlays U, which wave_y has consumed
+        planes_of_M<KL::PLANAR>(S.Mv, S.Mb);  // Mb overlays U, which wave_y has consumed
         __syncthreads();
         uint32_t cols[NCL][4][NG];
         build_columns(S, cols);
@@ -857,13 +915,14 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
 
 // Batch form, attempt 0, part 2: columns of [A | y - A r], Gauss-Jordan,
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
-template <class Acc>
+// PL: the M vectors in the workspace are planar (prepared keys).
+template <class Acc, bool PL>
 __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0) {
     __shared__ SolveShared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
-    planes_of_M((const uint32_t *)(it.scratch + SCR_MV), S.Mb);
+    planes_of_M<PL>((const uint32_t *)(it.scratch + SCR_MV), S.Mb);
     const uint32_t *yv = (const uint32_t *)(it.scratch + SCR_Y);
     for (int e = lane; e < MW; e += 64) S.y[e] = yv[e];
     for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);

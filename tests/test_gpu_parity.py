@@ -167,9 +167,21 @@ def test_device_key_tables_match_oracle(engine_factory):
             assert epk[i].tobytes() == O.expandpk(v, opk)
 
 
+def _planar_words(entries):
+    """Each little-endian dword of the entries (last axis, bytes) with bit
+    4e + d moved to bit 8d + e: byte d then holds bit d of the 8 nibbles."""
+    w = np.ascontiguousarray(entries).view("<u4")
+    out = np.zeros_like(w)
+    for e in range(8):
+        for d in range(4):
+            out |= ((w >> np.uint32(4 * e + d)) & np.uint32(1)) << np.uint32(8 * d + e)
+    return out.view(np.uint8)
+
+
 def _prepared_layout(z, table, kind):
     """numpy restatement of the prepared layout: entries padded to a
-    multiple of 4 bytes, blocks 16-byte aligned, rows a multiple of 128."""
+    multiple of 4 bytes and stored planar, blocks 16-byte aligned, rows a
+    multiple of 128."""
     mb = z.m // 2
     mbp = (mb + 3) // 4 * 4
     v, o = z.n - z.o, z.o
@@ -189,8 +201,9 @@ def _prepared_layout(z, table, kind):
     d = dst0
     for s0, cnt in blocks:
         src = table[:, s0:s0 + cnt * mb].reshape(-1, cnt, mb)
-        dst = out[:, d:d + cnt * mbp].reshape(-1, cnt, mbp)
-        dst[:, :, :mb] = src
+        padded = np.zeros((table.shape[0], cnt, mbp), np.uint8)
+        padded[:, :, :mb] = src
+        out[:, d:d + cnt * mbp] = _planar_words(padded).reshape(table.shape[0], cnt * mbp)
         d += cnt * mbp
     return out
 
