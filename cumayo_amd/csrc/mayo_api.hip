@@ -146,7 +146,7 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
-    wv::k_sign_wave<PitchedSign><<<(unsigned)n, 64, 0, stream>>>(acc, n, 0);
+    wv::k_sign_wave<PitchedSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }
 
@@ -366,7 +366,8 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             const int fail_first = (stages >> 8) & 0xFF;
             wv::k_sign_prep<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
             wv::k_sign_solve<TableSign, KL::PLANAR><<<(unsigned)n, 64, 0, stream>>>(acc, n, fail_first);
-            wv::k_sign_wave<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n, 1, fail_first);
+            const unsigned retry_grid = (unsigned)(n < 256 ? n : 256);
+            wv::k_sign_retry<TableSign, KL><<<retry_grid, 64 * wv::RNW, 0, stream>>>(acc, n, fail_first);
         }
         e = hipGetLastError();
     }

@@ -34,9 +34,12 @@ constexpr int SCR_T = DIGEST + SK;
 constexpr int SCR_VR = DIGEST + SK + MB;
 constexpr int SCR_ATTEMPTS = SCR_VR + VR_BYTES;  // attempts used (0 = all failed)
 constexpr int SIGN_SCRATCH = SCR_ATTEMPTS + 1;
-// Batch form only: the M products and y of attempt 0, handed from the
-// product kernel to the solve kernel.
-constexpr int SCR_MV = (SIGN_SCRATCH + 63) / 64 * 64;
+// Batch form only: the list of items whose attempt 0 failed (entry k in
+// item k's scratch, the count in item 0's), and the M products and y of
+// attempt 0, handed from the product kernel to the solve kernel.
+constexpr int SCR_LIST = (SIGN_SCRATCH + 3) / 4 * 4;
+constexpr int SCR_LIST_COUNT = SCR_LIST + 4;
+constexpr int SCR_MV = (SCR_LIST_COUNT + 4 + 63) / 64 * 64;
 constexpr int SCR_Y = SCR_MV + K * O * MW * 4;
 constexpr int SIGN_SCRATCH_STRIDE = (SCR_Y + MW * 4 + 63) / 64 * 64;
 constexpr uint8_t ATTEMPT_PENDING = 0xFF;   // rank failure left for the retry pass
@@ -83,6 +86,7 @@ struct TableSign {
                 (uint32_t)msg_len[i], sigs + i * (size_t)SIG_BYTES,
                 scratch + i * (size_t)SIGN_SCRATCH_STRIDE};
     }
+    __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)SIGN_SCRATCH_STRIDE; }
 };
 
 struct VerifyItem {

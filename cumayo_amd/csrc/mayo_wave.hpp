@@ -73,6 +73,18 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | (uint64_t)uni((uint32_t)x);
 }
 
+// Barrier policies of the helpers below: the whole workgroup, or only the
+// one wave that runs them while the others wait at the caller's next
+// barrier. A wave's LDS accesses complete in program order, so one wave
+// needs only a fence (no compiler reordering across it, own LDS operations
+// complete).
+struct BlockSync {
+    __device__ static void sync() { __syncthreads(); }
+};
+struct WaveSync {
+    __device__ static void sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+};
+
 // Multiply a bitsliced nibble vector (planes p0..p3) by a.
 __device__ __forceinline__ void bs_mul_a(uint32_t &p0, uint32_t &p1, uint32_t &p2, uint32_t &p3) {
     uint32_t t = p3;
@@ -224,16 +236,22 @@ struct EntryLoad {
 // Lanes = (word w, row slot); a chunk gives each slot QF_RT rows, rows in
 // ascending start column so that every row of a chunk starts within a few
 // columns of the first (those columns are masked per lane).
-template <class KL, int NLR, bool FULL>
+template <int NLR, bool FULL>
+constexpr int qf_chunks() {
+    return (NLR + (FULL ? N : V) + QF_RT * SL - 1) / (QF_RT * SL);
+}
+template <class KL, int NLR, bool FULL, int NW = 1>
 __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *scol, uint32_t *Ulds,
                                           uint32_t *Mv) {
     constexpr int NP = FULL ? N : V;          // P rows and columns
     constexpr int NVR = NLR + NP;             // rows of the schedule
     constexpr int RPC = QF_RT * SL;           // rows per chunk
-    constexpr int NCH = (NVR + RPC - 1) / RPC;
+    constexpr int NCH = qf_chunks<NLR, FULL>();
     constexpr int EB = KL::EB;
     constexpr uint32_t P1o = FULL ? KL::EPK_P1 : KL::ESK_P1;
-    const int lane = threadIdx.x;
+    // NW waves of the workgroup share the chunks (chunk ch goes to wave ch % NW)
+    const int lane = threadIdx.x & 63;
+    const int wave = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int w = lane % MW, slot = lane / MW;
     const bool active = slot < SL;
     const int nv = mvec::word_valid(w);
@@ -242,7 +260,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
 #pragma unroll
     for (int p = 0; p < PAIRS; ++p) U[p] = 0;
 #pragma unroll 1
-    for (int ch = 0; ch < NCH; ++ch) {
+    for (int ch = wave; ch < NCH; ch += NW) {
         const int v0 = ch * RPC;
         const int vlast = (v0 + RPC < NVR ? v0 + RPC : NVR) - 1;
         const int c_lo = start_of(v0), c_free = start_of(vlast);
@@ -384,7 +402,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
     }
     if constexpr (KL::PLANAR) {
         __syncthreads();
-        for (int e = lane; e < PAIRS * MW; e += 64) Ulds[e] = gf16::planar::to_nibbles(Ulds[e]);
+        for (int e = threadIdx.x; e < PAIRS * MW; e += 64 * NW) Ulds[e] = gf16::planar::to_nibbles(Ulds[e]);
     }
 }
 
@@ -448,6 +466,7 @@ __device__ __forceinline__ int ell_of(int i, int j) {
 }
 
 // y = t + sum_l E^l u_l computed by the wave; result in S.y (MW words).
+template <class Sync = BlockSync>
 __device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
     const int lane = threadIdx.x;
     if (lane < UW) {
@@ -463,7 +482,7 @@ __device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
             }
         S.yu[lane] = acc;
     }
-    __syncthreads();
+    Sync::sync();
     if (lane == 0) {
         uint32_t acc[UW], out[MW];
 #pragma unroll
@@ -472,7 +491,7 @@ __device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
 #pragma unroll
         for (int q = 0; q < MW; ++q) S.y[q] = out[q] ^ t[q];
     }
-    __syncthreads();
+    Sync::sync();
 }
 
 // GF(16) product of a per-lane nibble and a uniform nibble via the LDS table.
@@ -660,13 +679,13 @@ __device__ __forceinline__ void dispatch_word(int g, F &&f) {
 // Pivot columns are the greedy ones in column order, as in the reference's
 // echelon form (sign.cu:10-145), and z on them is y'[pr] / p: the solution
 // with the free variables at zero is unique, so x equals the reference's.
-template <class LDS>
+template <class Sync = BlockSync, class LDS>
 __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = false) {
     const int lane = threadIdx.x;
     const uint8_t *rv = S.vr + R_OFF;
     // y' = y - A r: every A column times its r nibble, XOR-reduced over lanes
     if (lane < 4 * NG) S.Ar[lane] = 0;
-    __syncthreads();
+    Sync::sync();
     {
         uint32_t part[4][NG];
 #pragma unroll
@@ -694,7 +713,7 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
             for (int g = 0; g < NG; ++g)
                 if (part[d][g]) atomicXor(&S.Ar[d * NG + g], part[d][g]);
     }
-    __syncthreads();
+    Sync::sync();
     if (lane == Y_LANE) {
 #pragma unroll
         for (int d = 0; d < 4; ++d)
@@ -789,7 +808,7 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
 #pragma unroll
             for (int g = 0; g < NG; ++g) S.yplanes[d * NG + g] = cols[Y_SET][d][g];
     }
-    __syncthreads();
+    Sync::sync();
     // force_fail (test hook): report a singular system; x stays r, as when
     // the rank check fails
     const int ok = rank == M && !force_fail;
@@ -804,7 +823,7 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
         }
         S.x[c] = (uint8_t)xv;
     }
-    __syncthreads();
+    Sync::sync();
     return ok;
 }
 
@@ -842,34 +861,29 @@ __device__ __forceinline__ void wave_write_sig(const LDS &S, const SignItem &it)
     for (int b = lane; b < SK; b += 64) it.sig[SIG_S_BYTES + b] = it.scratch[SCR_SALT + b];
 }
 
-// The whole signing loop in one wave (sign.cu:244-478). With retry_only set
-// it handles only items the solve kernel left pending, from attempt 1 on.
-// Attempts below fail_first report a singular system (test hook).
+// V || r of attempt c >= 1, re-derived by one wave (sign.cu:251-252):
+// SHAKE256(digest || salt || seed || c) into S.vr.
+template <class Sync>
+__device__ __forceinline__ void derive_vr(Shared &S, const SignItem &it, int c) {
+    const int lane = threadIdx.x & 63;
+    for (int b = lane; b < DIGEST + SK; b += 64) S.hbuf[b] = it.scratch[SCR_DIGEST + b];
+    for (int b = lane; b < SK; b += 64) S.hbuf[DIGEST + SK + b] = it.esk[b];
+    if (lane == 0) S.hbuf[DIGEST + 2 * SK] = (uint8_t)c;
+    Sync::sync();
+    keccak::wave_shake256(S.vr, VR_BYTES, S.hbuf, DIGEST + 2 * SK + 1);
+}
+
+// The whole signing loop in one wave (sign.cu:244-478), for the reference
+// pool. Attempts below fail_first report a singular system (test hook).
 template <class Acc, class KL = RefKeys>
-__global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int retry_only,
-                                                          int fail_first = 0) {
+__global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int fail_first = 0) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
-    if (retry_only && it.scratch[SCR_ATTEMPTS] != ATTEMPT_PENDING) return;
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
-    int ctr = retry_only ? 1 : 0;
-    // V || r of the first attempt handled here: from the hashing pass for
-    // attempt 0, re-derived for attempt ctr >= 1 (sign.cu:251-252)
-    auto derive_vr = [&](int c) {
-        for (int b = lane; b < DIGEST + SK; b += 64) S.hbuf[b] = it.scratch[SCR_DIGEST + b];
-        for (int b = lane; b < SK; b += 64) S.hbuf[DIGEST + SK + b] = it.esk[b];
-        if (lane == 0) S.hbuf[DIGEST + 2 * SK] = (uint8_t)c;
-        __syncthreads();
-        keccak::wave_shake256(S.vr, VR_BYTES, S.hbuf, DIGEST + 2 * SK + 1);
-    };
-    if (ctr == 0) {
-        for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
-    } else {
-        derive_vr(ctr);
-    }
-    int ok = 0;
+    for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
+    int ctr = 0, ok = 0;
     for (;; ++ctr) {
         __syncthreads();
         wave_scol(S);
@@ -884,11 +898,68 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int
         build_columns(S, cols);
         ok = solve(S, cols, ctr < fail_first);
         if (ok || ctr == MAX_ATTEMPTS - 1) break;
-        derive_vr(ctr + 1);
+        derive_vr<BlockSync>(S, it, ctr + 1);
     }
     if (lane == 0) it.scratch[SCR_ATTEMPTS] = ok ? (uint8_t)(ctr + 1) : 0;
     __syncthreads();
     wave_write_sig(S, it);
+}
+
+// wave 0 of the retry kernel: y, the columns of A, the elimination (a call
+// of its own, which keeps the register use of the kernel down)
+template <bool PL>
+__device__ __noinline__ void retry_linear_algebra(Shared &S, int ctr, bool force_fail, int *state) {
+    wave_y<WaveSync>(S, S.t);
+    planes_of_M<PL>(S.Mv, S.Mb);
+    WaveSync::sync();
+    uint32_t cols[NCL][4][NG];
+    build_columns(S, cols);
+    const int ok = solve<WaveSync>(S, cols, force_fail);
+    if (threadIdx.x == 0) *state = ok ? 1 : (ctr == MAX_ATTEMPTS - 1 ? 2 : 0);
+}
+
+// Retry pass of the batch form: the items whose attempt 0 failed (about 1
+// in 5,000), listed by the solve kernel, from attempt 1 on. One workgroup of
+// RNW waves per item, so that one item's attempt takes a fraction of a
+// single wave's time: the waves share the row chunks of the quadratic form,
+// wave 0 re-derives V || r and runs the linear algebra. Grid: any size; a
+// workgroup takes list entries blockIdx.x, + gridDim.x, ...
+constexpr int RNW_MAX = M <= 80 ? 8 : 4;  // larger sets: a whole SIMD's registers per wave
+constexpr int RNW = qf_chunks<O, false>() < RNW_MAX ? qf_chunks<O, false>() : RNW_MAX;
+template <class Acc, class KL>
+__global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, int fail_first = 0) {
+    __shared__ Shared S;
+    __shared__ int state;  // 0 next attempt, 1 solved, 2 attempts exhausted
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t count = *(const uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT);
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const uint32_t item = *(const uint32_t *)(acc.scratch_of(k) + SCR_LIST);
+        SignItem it = acc(item);
+        if (wave == 0) {
+            for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
+            for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
+        }
+        int ctr = 1;
+        for (;; ++ctr) {
+            if (wave == 0) {
+                derive_vr<WaveSync>(S, it, ctr);
+                WaveSync::sync();
+                wave_scol(S);
+            }
+            for (int e = threadIdx.x; e < PAIRS * MW; e += 64 * RNW) S.U[e] = 0;
+            __syncthreads();
+            quad_form<KL, O, false, RNW>(it.esk, S.scol, S.U, S.Mv);
+            __syncthreads();
+            if (wave == 0) retry_linear_algebra<KL::PLANAR>(S, ctr, ctr < fail_first, &state);
+            __syncthreads();
+            if (state) break;
+        }
+        if (wave == 0) {
+            if (lane == 0) it.scratch[SCR_ATTEMPTS] = state == 1 ? (uint8_t)(ctr + 1) : 0;
+            wave_write_sig(S, it);
+        }
+        __syncthreads();
+    }
 }
 
 // Batch form, attempt 0, part 1: M, U and y, handed over through the
@@ -898,6 +969,8 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
+    // the failure list of the solve kernel starts empty
+    if (blockIdx.x == 0 && lane == 0) *(uint32_t *)(it.scratch + SCR_LIST_COUNT) = 0;
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     __syncthreads();
@@ -931,7 +1004,12 @@ __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n,
     build_columns(S, cols);
     const int ok = solve(S, cols, fail_first > 0);
     if (!ok) {
-        if (lane == 0) it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
+        // pending: listed for the retry pass
+        if (lane == 0) {
+            it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
+            const uint32_t k = atomicAdd((uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT), 1u);
+            *(uint32_t *)(acc.scratch_of(k) + SCR_LIST) = (uint32_t)blockIdx.x;
+        }
         return;
     }
     if (lane == 0) it.scratch[SCR_ATTEMPTS] = 1;

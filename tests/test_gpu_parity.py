@@ -155,6 +155,40 @@ def test_table_form_large_batch_includes_retries(engine_factory):
     assert np.array_equal(psigs.cpu().numpy(), g)
 
 
+@pytest.mark.parametrize("v", (1, 5), ids=lambda v: f"MAYO_{v}")
+def test_retry_pass_many_items(engine_factory, v):
+    """Forced rank failures of attempts 0 and 1 for 300 items: every item
+    goes through the retry pass (more items than its workgroups, so they
+    take several each) and comes out as the oracle's signature under the
+    same forced failures, with attempts >= 3."""
+    import torch
+    eng = engine_factory(v)
+    z = eng.sizes
+    n, nkeys, ff = 300, 3, 2
+    esk, _ = eng.keypair_tables(nkeys, rand_index=7)
+    pesk, _ = eng.prepare_tables(esk, None)
+    rng = np.random.default_rng(11 + v)
+    lens = rng.integers(0, 200, n).astype(np.int32)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+    kidx = (np.arange(n) % nkeys).astype(np.int32)
+    dev = eng.device
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sigs = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+    ws = eng.sign_workspace(n)
+    print(f"[parity] forced retries MAYO_{v} n={n}", flush=True)
+    eng.sign_prepared(pesk, t(kidx), t(flat), t(offs), t(lens), sigs, ws, stages=3 | (ff << 8))
+    torch.cuda.synchronize()
+    att = eng.sign_attempts(ws, n)
+    assert (att >= ff + 1).all(), att.min()
+    g = sigs.cpu().numpy()
+    esk_h = esk.cpu().numpy()
+    for i in list(range(0, n, 23)) + [n - 1]:
+        osig, oatt = O.sign(v, esk_h[kidx[i]].tobytes(), flat[offs[i]:offs[i] + lens[i]].tobytes(), fail_first=ff)
+        assert g[i].tobytes() == osig, f"item {i}"
+        assert int(att[i]) == oatt, f"item {i}: attempts {att[i]} vs {oatt}"
+
+
 def test_device_key_tables_match_oracle(engine_factory):
     """keypair_tables: keys generated and expanded without leaving the device."""
     for v in (1, 2):
