@@ -30,8 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process
 # (default 4); the serve runs keep up to 4 servers x 3 slot streams busy at
-# once, which on 4 queues serialise behind each other. Set before HIP starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# once, which on 4 queues serialise behind each other. Raised to 16 (also
+# over a smaller value already in the environment) before HIP starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 # Work per operation (GF(16) multiply-accumulates), from the reference loop
 # bounds: sign counts one attempt (sign.cu:256-437 plus the elimination),
@@ -384,7 +386,8 @@ def main():
                                     **point(0.5, args.serve_load * mixed_rate, args.serve_seconds)}
             st = srv.stats()
         serve["slots"] = {"max_batch": args.serve_batch, "n_slots": args.serve_slots,
-                          "batches": st["batches"], "overlapped_launches": st["overlapped_launches"]}
+                          "batches": st["batches"], "overlapped_launches": st["overlapped_launches"],
+                          "gpu_max_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])}
         if not args.no_mixed_sets:
             serve["mixed_sets"] = mixed_sets_run(eng, args)
         out["serve"] = serve

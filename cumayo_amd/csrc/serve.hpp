@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <random>
 #include <thread>
@@ -37,6 +38,95 @@ namespace serve {
 __global__ void k_gather_attempts(const uint8_t *__restrict__ ws, uint8_t *__restrict__ out, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = ws[i * mayo::SIGN_SCRATCH_STRIDE + mayo::SCR_ATTEMPTS];
+}
+
+// Fork-join pool for the host-side copies of a batch: run(n, chunk, fn)
+// calls fn(begin, end) over [0, n) in chunks taken by the pool's threads
+// and the caller, and returns when all are done. One job at a time.
+class HostPool {
+  public:
+    explicit HostPool(int threads) {
+        for (int i = 0; i < threads; ++i) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    template <class F>
+    void run(size_t n, size_t chunk, F &&fn) {
+        if (th_.empty() || n <= chunk) {
+            fn(size_t(0), n);
+            return;
+        }
+        const std::function<void(size_t, size_t)> job = fn;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &job;
+            n_ = n;
+            chunk_ = chunk;
+            next_.store(0);
+            busy_ = (int)th_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    void work() {
+        size_t b;
+        while ((b = next_.fetch_add(chunk_)) < n_) (*job_)(b, std::min(b + chunk_, n_));
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            // run() waits for every thread before it returns, so each
+            // thread takes part in each job once
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t, size_t)> *job_ = nullptr;
+    size_t n_ = 0, chunk_ = 1;
+    std::atomic<size_t> next_{0};
+    int busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Stable counting sort of requests by the SHAKE256 blocks their message
+// absorbs (136 bytes each; 255 and more share the last bin).
+inline void order_by_blocks(std::vector<cumayo_request> &v, std::vector<cumayo_request> &tmp) {
+    constexpr size_t BINS = 256;
+    if (v.size() < 2) return;
+    size_t pos[BINS] = {};
+    for (const auto &r : v) ++pos[std::min<size_t>(r.len / 136, BINS - 1)];
+    size_t sum = 0;
+    for (size_t b = 0; b < BINS; ++b) {
+        const size_t c = pos[b];
+        pos[b] = sum;
+        sum += c;
+    }
+    tmp.resize(v.size());
+    for (const auto &r : v) tmp[pos[std::min<size_t>(r.len / 136, BINS - 1)]++] = r;
+    v.swap(tmp);
 }
 
 inline int64_t now_ns() {
@@ -58,11 +148,15 @@ struct Slot {
     int32_t *d_len = nullptr, *d_key = nullptr;
     size_t n_sign = 0, n_verify = 0;
     std::vector<cumayo_request> reqs;  // sign requests first, then verifications
+    std::vector<cumayo_request> vreqs; // verifications while admitting
+    std::vector<cumayo_request> tmp;
+    std::vector<cumayo_completion> comps;  // completions of the batch
 };
 
 class Server {
   public:
-    explicit Server(const cumayo_serve_config &cfg) : cfg_(cfg) {
+    explicit Server(const cumayo_serve_config &cfg)
+        : cfg_(cfg), pool_(cfg.host_threads > 0 ? cfg.host_threads - 1 : 3) {
         if (cfg_.max_batch <= 0) cfg_.max_batch = 4096;
         if (cfg_.n_slots <= 0) cfg_.n_slots = 3;
         if (!cfg_.slot_msg_bytes) cfg_.slot_msg_bytes = (size_t)cfg_.max_batch * 1024;
@@ -101,6 +195,9 @@ class Server {
             dev_alloc(&s.d_wss, mb * (size_t)mayo::SIGN_SCRATCH_STRIDE);
             dev_alloc(&s.d_wsv, mb * (size_t)mayo::VERIFY_SCRATCH_STRIDE);
             s.reqs.reserve(mb);
+            s.vreqs.reserve(mb);
+            s.tmp.reserve(mb);
+            s.comps.resize(mb);
         }
         thread_ = std::thread([this] { worker(); });
     }
@@ -233,35 +330,46 @@ class Server {
     // Admit waiting requests into slot s and launch its batch.
     bool launch(Slot &s) {
         s.reqs.clear();
+        s.vreqs.clear();
         {
             std::lock_guard<std::mutex> lk(mu_);
-            size_t bytes = 0;
-            while (!pending_.empty() && s.reqs.size() < (size_t)cfg_.max_batch) {
+            size_t bytes = 0, taken = 0;
+            while (!pending_.empty() && taken < (size_t)cfg_.max_batch) {
                 const cumayo_request &r = pending_.front();
                 if (bytes + r.len > cfg_.slot_msg_bytes) break;
                 bytes += r.len;
-                s.reqs.push_back(r);
+                (r.op == CUMAYO_OP_SIGN ? s.reqs : s.vreqs).push_back(r);
                 pending_.pop_front();
+                ++taken;
             }
-            if (s.reqs.empty()) return false;
-            in_flight_requests_ += s.reqs.size();
+            if (!taken) return false;
+            in_flight_requests_ += taken;
         }
-        std::stable_partition(s.reqs.begin(), s.reqs.end(),
-                              [](const cumayo_request &r) { return r.op == CUMAYO_OP_SIGN; });
+        // sign requests first, then verifications, each grouped by message
+        // length (the hashing pass gives each lane one message, so a wave
+        // takes as long as its longest one)
+        order_by_blocks(s.reqs, s.tmp);
+        order_by_blocks(s.vreqs, s.tmp);
+        s.n_sign = s.reqs.size();
+        s.n_verify = s.vreqs.size();
+        s.reqs.insert(s.reqs.end(), s.vreqs.begin(), s.vreqs.end());
         const size_t n = s.reqs.size();
-        s.n_sign = 0;
-        while (s.n_sign < n && s.reqs[s.n_sign].op == CUMAYO_OP_SIGN) ++s.n_sign;
-        s.n_verify = n - s.n_sign;
         int64_t off = 0;
         for (size_t i = 0; i < n; ++i) {
             const cumayo_request &r = s.reqs[i];
-            if (r.len) std::memcpy(s.h_msgs + off, r.msg, r.len);
             s.h_off[i] = off;
             s.h_len[i] = (int32_t)r.len;
             s.h_key[i] = r.key;
             off += r.len;
-            if (i >= s.n_sign) std::memcpy(s.h_sig + i * mayo::SIG_BYTES, r.sig_in, mayo::SIG_BYTES);
         }
+        // the copies into the pinned staging, spread over the host pool
+        pool_.run(n, 256, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                const cumayo_request &r = s.reqs[i];
+                if (r.len) std::memcpy(s.h_msgs + s.h_off[i], r.msg, r.len);
+                if (i >= s.n_sign) std::memcpy(s.h_sig + i * mayo::SIG_BYTES, r.sig_in, mayo::SIG_BYTES);
+            }
+        });
         const hipStream_t st = s.stream;
         ok(hipMemcpyAsync(s.d_msgs, s.h_msgs, (size_t)off + 1, hipMemcpyHostToDevice, st));
         ok(hipMemcpyAsync(s.d_off, s.h_off, n * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -299,29 +407,31 @@ class Server {
 
     void finish(Slot &s) {
         const int64_t t = now_ns();
-        std::vector<cumayo_completion> done;
-        done.reserve(s.reqs.size());
-        for (size_t i = 0; i < s.reqs.size(); ++i) {
-            const cumayo_request &r = s.reqs[i];
-            uint8_t status;
-            if (i < s.n_sign) {
-                if (r.sig_out) std::memcpy(r.sig_out, s.h_sig + i * mayo::SIG_BYTES, mayo::SIG_BYTES);
-                status = s.h_att[i] == 0 ? 1 : 0;
-                if (r.result_out) *r.result_out = s.h_att[i];
-            } else {
-                status = s.h_res[i - s.n_sign];
-                if (r.result_out) *r.result_out = status;
+        const size_t n = s.reqs.size();
+        pool_.run(n, 256, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                const cumayo_request &r = s.reqs[i];
+                uint8_t status;
+                if (i < s.n_sign) {
+                    if (r.sig_out) std::memcpy(r.sig_out, s.h_sig + i * mayo::SIG_BYTES, mayo::SIG_BYTES);
+                    status = s.h_att[i] == 0 ? 1 : 0;
+                    if (r.result_out) *r.result_out = s.h_att[i];
+                } else {
+                    status = s.h_res[i - s.n_sign];
+                    if (r.result_out) *r.result_out = status;
+                }
+                s.comps[i] = {r.tag, t - r.t_arrival_ns, status};
             }
-            done.push_back({r.tag, t - r.t_arrival_ns, status});
-        }
+        });
         s.busy = false;
         std::lock_guard<std::mutex> lk(mu_);
-        completed_.insert(completed_.end(), done.begin(), done.end());
+        completed_.insert(completed_.end(), s.comps.begin(), s.comps.begin() + (long)n);
         requests_done_ += s.reqs.size();
         in_flight_requests_ -= s.reqs.size();
     }
 
     cumayo_serve_config cfg_;
+    HostPool pool_;
     uint8_t *pesk_ = nullptr, *pepk_ = nullptr;  // prepared key tables
     std::vector<Slot> slots_;
     std::mutex mu_;

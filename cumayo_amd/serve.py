@@ -25,7 +25,8 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("esk_table", _VP), ("esk_stride", _SZ),
                 ("epk_table", _VP), ("epk_stride", _SZ), ("n_keys", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("n_slots", ctypes.c_int32),
-                ("slot_msg_bytes", _SZ), ("max_queue", ctypes.c_int64)]
+                ("slot_msg_bytes", _SZ), ("max_queue", ctypes.c_int64),
+                ("host_threads", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -80,7 +81,8 @@ def bind(lib: ctypes.CDLL) -> None:
 
 class Server:
     def __init__(self, engine, esk_table=None, epk_table=None, max_batch: int = 4096,
-                 n_slots: int = 3, slot_msg_bytes: int | None = None, max_queue: int = 0):
+                 n_slots: int = 3, slot_msg_bytes: int | None = None, max_queue: int = 0,
+                 host_threads: int = 0):
         self.eng = engine
         self.lib = engine.lib
         bind(self.lib)
@@ -99,7 +101,8 @@ class Server:
             epk_table=epk_table.data_ptr() if epk_table is not None else None,
             epk_stride=epk_table.stride(0) if epk_table is not None else 0,
             n_keys=n_keys, max_batch=max_batch, n_slots=n_slots,
-            slot_msg_bytes=slot_msg_bytes or max_batch * 1024, max_queue=max_queue)
+            slot_msg_bytes=slot_msg_bytes or max_batch * 1024, max_queue=max_queue,
+            host_threads=host_threads)
         engine.torch.cuda.synchronize(engine.device)
         self.handle = self.lib.cumayo_serve_create(ctypes.byref(cfg))
         if not self.handle:

@@ -220,6 +220,8 @@ typedef struct {
     int32_t n_slots;        /* batches in flight */
     size_t slot_msg_bytes;  /* message bytes per slot */
     int64_t max_queue;      /* pending requests beyond which submit refuses (0 = no limit) */
+    int32_t host_threads;   /* threads that copy requests into and results out of the
+                               pinned staging (0 = 4; 1 = the worker alone) */
 } cumayo_serve_config;
 
 typedef struct {
