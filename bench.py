@@ -126,7 +126,7 @@ def mixed_sets_run(eng1, args, weights=None, pool=8192, keys=64, loads=(0.25, 0.
     capacity = 1.0 / per_req
     points = []
     with MixedServer(engines, tables, max_batch=pool, n_slots=3, slot_msg_bytes=pool * 4096) as ms:
-        ms.run_poisson(loads[0] * capacity, 0.3, weights, 0.5, pools)  # unrecorded warm-up
+        ms.run_poisson(0.9 * capacity, 0.3, weights, 0.5, pools)  # unrecorded warm-up, all slots busy
         for load in loads:
             print(f"[bench] serve mixed sets at {load:.0%} of {capacity:.0f}/s", file=sys.stderr, flush=True)
             rep = ms.run_poisson(load * capacity, seconds, weights, 0.5, pools)
@@ -163,8 +163,8 @@ def main():
                     help="length of each open-loop serve run (0 skips the serve runs)")
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="offered Poisson rate as a fraction of the measured batch throughput")
-    ap.add_argument("--serve-batch", type=int, default=16384, help="requests per serve slot")
-    ap.add_argument("--serve-slots", type=int, default=3, help="serve batches in flight")
+    ap.add_argument("--serve-batch", type=int, default=8192, help="requests per serve slot")
+    ap.add_argument("--serve-slots", type=int, default=6, help="serve batches in flight")
     ap.add_argument("--no-mixed-sets", action="store_true",
                     help="skip the mixed-parameter-set serve run (MAYO_1/2/3/5 = 40/20/20/20)")
     ap.add_argument("--flow-batch", type=int, default=20000,
@@ -370,9 +370,10 @@ def main():
                         "p50_ms": round(rep["p50_ms"], 3), "p90_ms": round(rep["p90_ms"], 3),
                         "p99_ms": round(rep["p99_ms"], 3), "max_ms": round(rep["max_ms"], 3),
                         "completed": rep["completed"], "refused": rep["refused"]}
-            # a short unrecorded run first: the first batches through a new
-            # server pay one-time costs (first launches, first-touch of pinned pages)
-            srv.run_poisson(0.3 * sign_per_s, 0.3, 0.0, 3, flat, offs, lens, kidx, sigs)
+            # a short unrecorded run first, at a load that keeps every slot
+            # busy: the first batches through each slot pay one-time costs
+            # (first launches, first-touch of pinned pages)
+            srv.run_poisson(0.9 * sign_per_s, 0.3, 0.0, 3, flat, offs, lens, kidx, sigs)
             serve["sign_vs_load"] = []
             for frac in sweep:
                 print(f"[bench] serve sign at {frac:.0%} of batch throughput", file=sys.stderr, flush=True)

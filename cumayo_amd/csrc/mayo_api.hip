@@ -354,7 +354,13 @@ template <class KL>
 int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stages) {
     hipError_t e = hipSuccess;
     if (stages & 1) {
-        k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        // lanes of a hashing wave take messages of similar length
+        if (n > 64) {
+            k_length_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
+            k_sign_hash<TableSign, true><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        } else {
+            k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess && (stages & 2)) {
@@ -379,7 +385,12 @@ template <class KL>
 int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int stages) {
     hipError_t e = hipSuccess;
     if (stages & 1) {
-        k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        if (n > 64) {
+            k_length_order<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
+            k_verify_hash<TableVerify, true><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        } else {
+            k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess && (stages & 2)) {

@@ -39,12 +39,21 @@ constexpr int SIGN_SCRATCH = SCR_ATTEMPTS + 1;
 // attempt 0, handed from the product kernel to the solve kernel.
 constexpr int SCR_LIST = (SIGN_SCRATCH + 3) / 4 * 4;
 constexpr int SCR_LIST_COUNT = SCR_LIST + 4;
-constexpr int SCR_MV = (SCR_LIST_COUNT + 4 + 63) / 64 * 64;
+constexpr int SCR_ORDER = SCR_LIST_COUNT + 4;  // hashing order (entry k in item k's scratch)
+constexpr int SCR_MV = (SCR_ORDER + 4 + 63) / 64 * 64;
 constexpr int SCR_Y = SCR_MV + K * O * MW * 4;
 constexpr int SIGN_SCRATCH_STRIDE = (SCR_Y + MW * 4 + 63) / 64 * 64;
 constexpr uint8_t ATTEMPT_PENDING = 0xFF;   // rank failure left for the retry pass
-// Scratch per verifying item: t.
-constexpr int VERIFY_SCRATCH_STRIDE = (MB + 15) / 16 * 16;
+// Scratch per verifying item: t, and the hashing order slot.
+constexpr int VSCR_ORDER = (MB + 3) / 4 * 4;
+constexpr int VERIFY_SCRATCH_STRIDE = (VSCR_ORDER + 4 + 15) / 16 * 16;
+// Hashing order: items grouped by the SHAKE256 blocks (136 bytes) their
+// message absorbs; ORDER_BINS - 1 and more share the last group.
+constexpr int ORDER_BINS = 256;
+__device__ __forceinline__ uint32_t order_bin(int32_t len) {
+    const uint32_t b = (uint32_t)len / 136u;
+    return b < ORDER_BINS - 1 ? b : ORDER_BINS - 1;
+}
 
 // ---------------------------------------------------------------------------
 // Item accessors
@@ -87,6 +96,9 @@ struct TableSign {
                 scratch + i * (size_t)SIGN_SCRATCH_STRIDE};
     }
     __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)SIGN_SCRATCH_STRIDE; }
+    __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
+    __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + SCR_ORDER) = i; }
+    __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + SCR_ORDER); }
 };
 
 struct VerifyItem {
@@ -123,6 +135,10 @@ struct TableVerify {
                 (uint32_t)msg_len[i], sigs + i * (size_t)SIG_BYTES, results + i,
                 scratch ? scratch + i * (size_t)VERIFY_SCRATCH_STRIDE : nullptr};
     }
+    __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)VERIFY_SCRATCH_STRIDE; }
+    __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
+    __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + VSCR_ORDER) = i; }
+    __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + VSCR_ORDER); }
 };
 
 // ---------------------------------------------------------------------------
@@ -169,11 +185,36 @@ __global__ void __launch_bounds__(64) k_seed(uint8_t *pool, size_t pitch, size_t
     }
 }
 
-// digest, salt (R = 0), t and V || r for ctr = 0 (sign.cu:224-252).
+// Hashing order for the lane-per-item hashing kernels below: items grouped
+// by the SHAKE256 blocks their message absorbs, so that the lanes of a wave
+// hash messages of similar length (a wave lasts as long as its longest
+// message). Counting sort in one workgroup; entry k goes to acc.set_order(k).
 template <class Acc>
+__global__ void __launch_bounds__(1024) k_length_order(Acc acc, size_t n) {
+    __shared__ uint32_t pos[ORDER_BINS];
+    for (int b = threadIdx.x; b < ORDER_BINS; b += 1024) pos[b] = 0;
+    __syncthreads();
+    for (size_t i = threadIdx.x; i < n; i += 1024) atomicAdd(&pos[acc.bin(i)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int b = 0; b < ORDER_BINS; ++b) {
+            const uint32_t c = pos[b];
+            pos[b] = sum;
+            sum += c;
+        }
+    }
+    __syncthreads();
+    for (size_t i = threadIdx.x; i < n; i += 1024) acc.set_order(atomicAdd(&pos[acc.bin(i)], 1u), (uint32_t)i);
+}
+
+// digest, salt (R = 0), t and V || r for ctr = 0 (sign.cu:224-252). With
+// ORDERED, thread k hashes item acc.order(k).
+template <class Acc, bool ORDERED = false>
 __global__ void __launch_bounds__(64) k_sign_hash(Acc acc, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if constexpr (ORDERED) i = acc.order(i);
     SignItem it = acc(i);
     uint8_t *s = it.scratch;
     keccak::Shake<136> h;
@@ -215,10 +256,11 @@ __device__ __forceinline__ void verify_target(const uint8_t *msg, uint32_t mlen,
     h.squeeze(t, MB);
 }
 
-template <class Acc>
+template <class Acc, bool ORDERED = false>
 __global__ void __launch_bounds__(64) k_verify_hash(Acc acc, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if constexpr (ORDERED) i = acc.order(i);
     VerifyItem it = acc(i);
     uint8_t digest[DIGEST];
     verify_target(it.msg, it.mlen, it.sig, digest, it.t);
