@@ -191,10 +191,18 @@ __global__ void __launch_bounds__(64) k_seed(uint8_t *pool, size_t pitch, size_t
 // message). Counting sort in one workgroup; entry k goes to acc.set_order(k).
 template <class Acc>
 __global__ void __launch_bounds__(1024) k_length_order(Acc acc, size_t n) {
+    constexpr int U = 8;  // items per thread per step: their loads are in flight together
     __shared__ uint32_t pos[ORDER_BINS];
     for (int b = threadIdx.x; b < ORDER_BINS; b += 1024) pos[b] = 0;
     __syncthreads();
-    for (size_t i = threadIdx.x; i < n; i += 1024) atomicAdd(&pos[acc.bin(i)], 1u);
+    for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
+        uint32_t bin[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bin[u] = i0 + 1024 * u < n ? acc.bin(i0 + 1024 * u) : ORDER_BINS;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (bin[u] < ORDER_BINS) atomicAdd(&pos[bin[u]], 1u);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t sum = 0;
@@ -205,7 +213,16 @@ __global__ void __launch_bounds__(1024) k_length_order(Acc acc, size_t n) {
         }
     }
     __syncthreads();
-    for (size_t i = threadIdx.x; i < n; i += 1024) acc.set_order(atomicAdd(&pos[acc.bin(i)], 1u), (uint32_t)i);
+    for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
+        uint32_t bin[U], k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bin[u] = i0 + 1024 * u < n ? acc.bin(i0 + 1024 * u) : ORDER_BINS;
+#pragma unroll
+        for (int u = 0; u < U; ++u) k[u] = bin[u] < ORDER_BINS ? atomicAdd(&pos[bin[u]], 1u) : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (bin[u] < ORDER_BINS) acc.set_order(k[u], (uint32_t)(i0 + 1024 * u));
+    }
 }
 
 // digest, salt (R = 0), t and V || r for ctr = 0 (sign.cu:224-252). With

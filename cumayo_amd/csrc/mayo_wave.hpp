@@ -793,9 +793,13 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                             cols[s][0][g] = c0; cols[s][1][g] = c1; cols[s][2][g] = c2; cols[s][3][g] = c3;
                         }
                     }
-                    used[pg] |= 1u << pb;
                     S.pinv[pc] = (uint8_t)im[0];
                 });
+                // outside the specialised step, which would otherwise keep
+                // `used` in scratch memory
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    if (g == (pr >> 5)) used[g] |= 1u << pb;
                 ++rank;
             }
             S.prow[pc] = (int16_t)pr;  // same value from every lane
