@@ -303,7 +303,14 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                 off[tt] += stride[tt];
             }
         };
-        auto column = [&](int c, int k, bool masked, bool more) {
+        // the rows of slot group tt start at column tt_start[tt] or later
+        // (uniform): left of the last group's start fewer groups are summed
+        int tt_start[QF_RT];
+#pragma unroll
+        for (int tt = 0; tt < QF_RT; ++tt) tt_start[tt] = v0 + tt * SL < NVR ? start_of(v0 + tt * SL) : NP;
+        // groups 0 .. NTT-1 take column c
+        auto column = [&](auto NT, int c, int k, bool masked, bool more) {
+            constexpr int NTT = decltype(NT)::value;
             const uint64_t sc = uni64(scol[c]);
             // masks from the 32-bit halves: one scalar bit-field extract each
             const uint32_t sh[2] = {(uint32_t)sc, (uint32_t)(sc >> 32)};
@@ -321,19 +328,35 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                     const int bit = 4 * j + b;
                     const uint32_t m = gf16::bitmask(sh[bit >> 5], bit & 31);
 #pragma unroll
-                    for (int tt = 0; tt < QF_RT; ++tt) Tb[tt][j][b] = gf16::axm(Tb[tt][j][b], x[tt], m);
+                    for (int tt = 0; tt < NTT; ++tt) Tb[tt][j][b] = gf16::axm(Tb[tt][j][b], x[tt], m);
                 }
         };
+        using All = std::integral_constant<int, QF_RT>;
         // columns [c0, cend), masked below cm
         auto segment = [&](int c0, int cm, int cend) {
 #pragma unroll
             for (int k = 0; k < QF_PF; ++k)
                 if (c0 + k < cend) fetch(k);
+            if constexpr (QF_PF == 1) {
+                // three loops: only group 0 started (QF_RT = 2), all groups
+                // masked, unmasked
+                int c = c0;
+                if constexpr (QF_RT == 2) {
+                    const int c1 = tt_start[1] < cm ? tt_start[1] : cm;
 #pragma unroll 1
-            for (int c = c0; c < cend; c += QF_PF) {
+                    for (; c < c1; ++c) column(std::integral_constant<int, 1>{}, c, 0, true, c + 1 < cend);
+                }
+#pragma unroll 1
+                for (; c < cm; ++c) column(All{}, c, 0, true, c + 1 < cend);
+#pragma unroll 1
+                for (; c < cend; ++c) column(All{}, c, 0, false, c + 1 < cend);
+            } else {
+#pragma unroll 1
+                for (int c = c0; c < cend; c += QF_PF) {
 #pragma unroll
-                for (int k = 0; k < QF_PF; ++k)
-                    if (c + k < cend) column(c + k, k, c + k < cm, c + k + QF_PF < cend);
+                    for (int k = 0; k < QF_PF; ++k)
+                        if (c + k < cend) column(All{}, c + k, k, c + k < cm, c + k + QF_PF < cend);
+                }
             }
         };
         // columns < V (P1 and L entries)
