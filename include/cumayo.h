@@ -157,9 +157,11 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
  * epk blocks (P1 || P2 || P3) with every m-vector entry padded with zero
  * bytes to a multiple of 4 bytes and every block 16-byte aligned, rows
  * cumayo_prepared_esk_bytes() / cumayo_prepared_epk_bytes() apart (multiples
- * of 128). Built once per key on the device from a reference-format table;
- * the kernels then read each entry word with one aligned load. `prepared`
- * must be 16-byte aligned; n_keys <= 65535 per call. */
+ * of 128), and each little-endian entry dword stored bit-plane-major (bit
+ * 4e + d of the nibble form at bit 8d + e: byte d holds bit d of the 8
+ * elements). Built once per key on the device from a reference-format table;
+ * the kernels then read each entry word with one aligned load. An opaque
+ * format: `prepared` must be 16-byte aligned; n_keys <= 65535 per call. */
 size_t cumayo_prepared_esk_bytes(void);
 size_t cumayo_prepared_epk_bytes(void);
 int cumayo_prepare_esk(const uint8_t *esk_table, size_t esk_stride, size_t n_keys, uint8_t *prepared,
