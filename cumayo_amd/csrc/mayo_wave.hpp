@@ -32,6 +32,7 @@ constexpr int NCOL = KO + 1;                   // columns of [A | y']
 constexpr int NCL = (NCOL + 63) / 64;          // columns held per lane
 constexpr int OVW = (PAIRS + 7) / 8 + 1;       // overflow words of an unreduced sum
 constexpr int UW = MW + OVW;                   // words of an unreduced sum
+constexpr int YG_MAX = 64 / UW;                // pair groups of the y sum (wave_y)
 // quad-form rows per lane per chunk, and the waves per SIMD the quad-form
 // kernels are compiled for (VGPR budget 512 / QF_OCC for the pair
 // accumulators and QF_RT x 4K bit-plane accumulators). Two rows and three
@@ -107,7 +108,7 @@ struct Shared {
         uint32_t U[PAIRS * MW];       // pair sums (quadratic form, then y)
         uint32_t Mb[KO * 4 * NG];     // bit planes of the M_j[x] (columns of A)
     };
-    uint32_t yu[UW];             // unreduced y
+    uint32_t yu[YG_MAX * UW];    // unreduced y (partial sums of the pair groups)
     uint32_t y[MW];              // y = t + sum E^l u_l
     uint32_t t[MW];
     uint32_t Ar[4 * NG];         // A r, bitsliced
@@ -483,29 +484,53 @@ __device__ __forceinline__ void to_planes(const uint32_t (&v)[MW], uint32_t (&pl
 }
 
 // Horner order of the pair (i, j): l(i, j) in the A schedule (sign.cu:336-392).
-__device__ __forceinline__ int ell_of(int i, int j) {
+__host__ __device__ constexpr int ell_of(int i, int j) {
     // rows i' < i contribute K - i' pairs each; within row i, j runs K-1 down
     return i * K - i * (i - 1) / 2 + (K - 1 - j);
+}
+
+// Unreduced sum_l z^l u_l over the pair sums U (nibble-packed, MW words per
+// pair, pair p = pair_index(i, j) shifted by l = ell_of(i, j)). Lanes are
+// (word q of the sum, group g of YPG consecutive pairs); the groups' partial
+// sums meet in ypart (YG * UW words), whose first UW words hold the sum on
+// return.
+constexpr int YG = YG_MAX;
+constexpr int YPG = (PAIRS + YG - 1) / YG;
+template <class Sync>
+__device__ __forceinline__ void unreduced_y(const uint32_t *U, uint32_t *ypart) {
+    const int lane = threadIdx.x & 63, q = lane % UW, g = lane / UW;
+    if (g < YG) {
+        // first pair (i, j) of the group
+        int p = g * YPG, i = 0;
+        while (pair_index(i + 1, i + 1) <= p && i + 1 < K) ++i;
+        int j = i + (p - pair_index(i, i));
+        uint32_t acc = 0;
+        for (int k = 0; k < YPG && p < PAIRS; ++k, ++p) {
+            const int ell = ell_of(i, j), ws = ell >> 3, ns = 4 * (ell & 7);
+            const uint32_t *src = U + p * MW;
+            const int hi = q - ws, lo = q - ws - 1;
+            const uint32_t h = (hi >= 0 && hi < MW) ? src[hi] : 0u;
+            const uint32_t l = (lo >= 0 && lo < MW) ? src[lo] : 0u;
+            acc ^= ns ? __builtin_amdgcn_alignbit(h, l, 32 - ns) : h;
+            if (++j == K) j = ++i;
+        }
+        ypart[g * UW + q] = acc;
+    }
+    Sync::sync();
+    if (lane < UW) {
+        uint32_t a = ypart[lane];
+#pragma unroll
+        for (int gg = 1; gg < YG; ++gg) a ^= ypart[gg * UW + lane];
+        ypart[lane] = a;
+    }
+    Sync::sync();
 }
 
 // y = t + sum_l E^l u_l computed by the wave; result in S.y (MW words).
 template <class Sync = BlockSync>
 __device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
-    const int lane = threadIdx.x;
-    if (lane < UW) {
-        uint32_t acc = 0;
-        for (int i = 0; i < K; ++i)
-            for (int j = i; j < K; ++j) {
-                const int ell = ell_of(i, j), ws = ell >> 3, ns = 4 * (ell & 7);
-                const uint32_t *src = &S.U[pair_index(i, j) * MW];
-                const int hi = lane - ws, lo = lane - ws - 1;
-                uint32_t h = (hi >= 0 && hi < MW) ? src[hi] : 0u;
-                uint32_t l = (lo >= 0 && lo < MW) ? src[lo] : 0u;
-                acc ^= ns ? __builtin_amdgcn_alignbit(h, l, 32 - ns) : h;
-            }
-        S.yu[lane] = acc;
-    }
-    Sync::sync();
+    const int lane = threadIdx.x & 63;
+    unreduced_y<Sync>(S.U, S.yu);
     if (lane == 0) {
         uint32_t acc[UW], out[MW];
 #pragma unroll
@@ -1223,7 +1248,7 @@ template <class Acc, class KL = RefKeys>
 __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
     __shared__ uint64_t scol[N];
     __shared__ uint32_t U[PAIRS * MW];
-    __shared__ uint32_t yu[UW];
+    __shared__ uint32_t yu[YG_MAX * UW];
     __shared__ uint8_t tb[MB + 4];
     __shared__ uint8_t hbuf[DIGEST + SK];
     const int lane = threadIdx.x;
@@ -1246,20 +1271,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
     __syncthreads();
     quad_form<KL, 0, true>(it.epk, scol, U, nullptr);
     __syncthreads();
-    if (lane < UW) {
-        uint32_t a = 0;
-        for (int i = 0; i < K; ++i)
-            for (int j = i; j < K; ++j) {
-                const int ell = ell_of(i, j), ws = ell >> 3, ns = 4 * (ell & 7);
-                const uint32_t *src = &U[pair_index(i, j) * MW];
-                const int hi = lane - ws, lo = lane - ws - 1;
-                uint32_t h = (hi >= 0 && hi < MW) ? src[hi] : 0u;
-                uint32_t l = (lo >= 0 && lo < MW) ? src[lo] : 0u;
-                a ^= ns ? __builtin_amdgcn_alignbit(h, l, 32 - ns) : h;
-            }
-        yu[lane] = a;
-    }
-    __syncthreads();
+    unreduced_y<BlockSync>(U, yu);
     if (lane == 0) {
         uint32_t a[UW], y[MW];
 #pragma unroll
