@@ -44,12 +44,7 @@ constexpr int YG_MAX = 64 / UW;                // pair groups of the y sum (wave
 #ifndef MAYO_QF_OCC
 #define MAYO_QF_OCC 0
 #endif
-#ifndef MAYO_QF_PF
-#define MAYO_QF_PF 0
-#endif
 constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : 2;
-// key-entry columns whose loads are in flight ahead of the one being summed
-constexpr int QF_PF = MAYO_QF_PF ? MAYO_QF_PF : 1;
 constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : 3);
 // waves per SIMD of the solve kernel (the bitsliced columns of [A | y'])
 constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
@@ -294,13 +289,12 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                 for (int b = 0; b < 4; ++b) Tb[tt][j][b] = 0;
         }
         // one column: every row of the chunk takes its entry at c, fetched
-        // QF_PF columns earlier into ring slot k (columns c0 + k, c0 + k +
-        // QF_PF, ... of a segment); off[tt] is the next column to fetch
-        EntryLoad<KL> ld[QF_PF][QF_RT];
-        auto fetch = [&](int k) {
+        // during the previous column; off[tt] is the next column to fetch
+        EntryLoad<KL> ld[QF_RT];
+        auto fetch = [&]() {
 #pragma unroll
             for (int tt = 0; tt < QF_RT; ++tt) {
-                ld[k][tt].fetch(key, off[tt], nv);
+                ld[tt].fetch(key, off[tt], nv);
                 off[tt] += stride[tt];
             }
         };
@@ -310,7 +304,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
 #pragma unroll
         for (int tt = 0; tt < QF_RT; ++tt) tt_start[tt] = v0 + tt * SL < NVR ? start_of(v0 + tt * SL) : NP;
         // groups 0 .. NTT-1 take column c
-        auto column = [&](auto NT, int c, int k, bool masked, bool more) {
+        auto column = [&](auto NT, int c, bool masked, bool more) {
             constexpr int NTT = decltype(NT)::value;
             const uint64_t sc = uni64(scol[c]);
             // masks from the 32-bit halves: one scalar bit-field extract each
@@ -318,10 +312,10 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
             uint32_t x[QF_RT];
 #pragma unroll
             for (int tt = 0; tt < QF_RT; ++tt) {
-                x[tt] = ld[k][tt].word(nv);
+                x[tt] = ld[tt].word(nv);
                 if (masked) x[tt] = c >= st[tt] ? x[tt] : 0u;
             }
-            if (more) fetch(k);
+            if (more) fetch();
 #pragma unroll
             for (int j = 0; j < K; ++j)
 #pragma unroll
@@ -333,32 +327,20 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                 }
         };
         using All = std::integral_constant<int, QF_RT>;
-        // columns [c0, cend), masked below cm
+        // columns [c0, cend), masked below cm, in three loops: only group 0
+        // started (QF_RT = 2), all groups masked, unmasked
         auto segment = [&](int c0, int cm, int cend) {
-#pragma unroll
-            for (int k = 0; k < QF_PF; ++k)
-                if (c0 + k < cend) fetch(k);
-            if constexpr (QF_PF == 1) {
-                // three loops: only group 0 started (QF_RT = 2), all groups
-                // masked, unmasked
-                int c = c0;
-                if constexpr (QF_RT == 2) {
-                    const int c1 = tt_start[1] < cm ? tt_start[1] : cm;
+            if (c0 < cend) fetch();
+            int c = c0;
+            if constexpr (QF_RT == 2) {
+                const int c1 = tt_start[1] < cm ? tt_start[1] : cm;
 #pragma unroll 1
-                    for (; c < c1; ++c) column(std::integral_constant<int, 1>{}, c, 0, true, c + 1 < cend);
-                }
-#pragma unroll 1
-                for (; c < cm; ++c) column(All{}, c, 0, true, c + 1 < cend);
-#pragma unroll 1
-                for (; c < cend; ++c) column(All{}, c, 0, false, c + 1 < cend);
-            } else {
-#pragma unroll 1
-                for (int c = c0; c < cend; c += QF_PF) {
-#pragma unroll
-                    for (int k = 0; k < QF_PF; ++k)
-                        if (c + k < cend) column(All{}, c + k, k, c + k < cm, c + k + QF_PF < cend);
-                }
+                for (; c < c1; ++c) column(std::integral_constant<int, 1>{}, c, true, c + 1 < cend);
             }
+#pragma unroll 1
+            for (; c < cm; ++c) column(All{}, c, true, c + 1 < cend);
+#pragma unroll 1
+            for (; c < cend; ++c) column(All{}, c, false, c + 1 < cend);
         };
         // columns < V (P1 and L entries)
         if (c_lo < V) segment(c_lo, c_free < V ? c_free : V, V);
