@@ -103,6 +103,31 @@ def test_serve_alone_equals_mix_equals_batch(engine_factory, version):
         assert mix[j][2].tobytes() == osig, f"oracle {j}"
 
 
+@pytest.mark.parametrize("host_threads", [1, 4])
+def test_serve_large_batches_host_pool(engine_factory, host_threads):
+    """Batches of several hundred requests, whose staging copies and result
+    copies are split over the host thread pool (chunks of 256 requests) or
+    done by the worker alone: the same bytes as the batch entry points."""
+    from cumayo_amd.serve import Server
+    eng = engine_factory(1)
+    nkeys, n = 4, 900
+    esk, epk = eng.keypair_tables(nkeys, rand_index=17)
+    reqs, batch_sigs, expect = _requests(eng, esk, epk, n, nkeys, seed=41)
+    print(f"[serve] large batches, host_threads={host_threads}", flush=True)
+    with Server(eng, esk, epk, max_batch=1024, n_slots=2, host_threads=host_threads) as srv:
+        assert srv.submit(reqs) == n
+        got = srv.wait(range(n))
+        st = srv.stats()
+    assert st["requests"] == n and st["batches"] <= 3
+    for j in range(n):
+        status, _, sig, res = got[j]
+        if reqs[j][1] == 0:
+            assert status == 0 and res >= 1
+            assert sig.tobytes() == batch_sigs[j].tobytes(), f"sign {j}"
+        else:
+            assert status == expect[j] and res == expect[j], f"verify {j}"
+
+
 def test_serve_refusals(engine_factory):
     from cumayo_amd.serve import STATUS_REFUSED, Server
     eng = engine_factory(1)
