@@ -170,6 +170,10 @@ def main():
     ap.add_argument("--flow-batch", type=int, default=20000,
                     help="batch of the reference-flow program (keypair, signWithesk, "
                          "verifyWithepk over 10 streams, 32-byte messages); 0 skips it")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the compiled default launch variants (no launch autotuning)")
+    ap.add_argument("--autotune-reps", type=int, default=3,
+                    help="timed runs per launch configuration in the autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -229,6 +233,16 @@ def main():
     def verify_step():
         eng.verify_prepared(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_v)
 
+    # launch autotuner: every compiled variant and item order of the
+    # linear-algebra kernels timed on this batch, the fastest kept (untimed,
+    # like the key preparation)
+    tune = None
+    if not args.no_autotune:
+        print("[bench] autotuning launch variants", file=sys.stderr, flush=True)
+        tune = eng.autotune(p_esk, p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_s, ws_v,
+                            reps=args.autotune_reps)
+        print(f"[bench] launch config {tune['best']}", file=sys.stderr, flush=True)
+
     for _ in range(args.warmup):
         sign_step()
         verify_step()
@@ -253,6 +267,31 @@ def main():
     verify_s = (time.perf_counter() - t0) / args.steps
     print(f"[bench] timed: sign {sign_s*1e3:.2f} ms/step verify {verify_s*1e3:.2f} ms/step",
           file=sys.stderr, flush=True)
+
+    # the same pipelines captured as HIP graphs (one graph launch per pass)
+    g_sigs = torch.zeros_like(d_sigs)
+    g_res = torch.full_like(d_res, 0xEE)
+    g_ws_s, g_ws_v = eng.sign_workspace(n), eng.verify_workspace(n)
+    g_sign = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, g_sigs, g_ws_s)
+    g_verify = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, g_res, g_ws_v)
+    g_sign.launch()
+    g_verify.launch()
+    torch.cuda.synchronize(dev)
+    graph_same = bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g_sign.launch()
+    torch.cuda.synchronize(dev)
+    graph_sign_s = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g_verify.launch()
+    torch.cuda.synchronize(dev)
+    graph_verify_s = (time.perf_counter() - t0) / args.steps
+    graph_same = graph_same and bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
+    g_sign.close()
+    g_verify.close()
+    del g_sigs, g_res, g_ws_s, g_ws_v
 
     # the plain form: one workgroup per item running the whole signing
     # (or verification) stage in one kernel, the reference's own structure
@@ -331,6 +370,10 @@ def main():
         "reference_layout_tables": {"sign_per_s": round(n / ref_sign_s, 1),
                                     "verify_per_s": round(n / ref_verify_s, 1),
                                     "identical_outputs": ref_same},
+        "graph": {"pipelines": "sign and verify passes captured as HIP graphs, one graph launch each",
+                  "sign_per_s": round(n / graph_sign_s, 1), "verify_per_s": round(n / graph_verify_s, 1),
+                  "identical_outputs": graph_same},
+        "launch_tuning": tune if tune is not None else {"best": eng.launch_config(), "autotuned": False},
         "sign_retries": {"items_with_retry": int((attempts > 1).sum()),
                          "exhausted": int((attempts == 0).sum()),
                          "max_attempts": int(attempts.max())},

@@ -350,8 +350,43 @@ size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_ST
 
 namespace {
 
+// Launch configuration of the batch form (process-wide for this library).
+// Over prepared keys, the compiled variant of each wave kernel, named by the
+// waves per SIMD it is built for; for every table form, the item order of
+// the one-wave-per-item kernels (0 batch order, 1 XCD-aware key order).
+struct LaunchCfg {
+    int prep, solve, sign_order, verify, verify_order;
+};
+std::atomic<int> g_prep{wv::PREP_OCCS[0]}, g_solve{wv::SOLVE_OCCS[0]}, g_sign_order{0},
+    g_verify{wv::VERIFY_OCCS[0]}, g_verify_order{0};
+
+LaunchCfg current_cfg() {
+    return {g_prep.load(), g_solve.load(), g_sign_order.load(), g_verify.load(), g_verify_order.load()};
+}
+
+template <size_t N>
+bool listed(const int (&arr)[N], int v) {
+    for (size_t i = 0; i < N; ++i)
+        if (arr[i] == v) return true;
+    return false;
+}
+
+// f(integral_constant<int, ARR[i]>) for the entry ARR[i] == want; false when
+// no compiled variant matches
+template <const auto &ARR, size_t I = 0, class F>
+bool with_occ(int want, F &&f) {
+    if constexpr (I < sizeof(ARR) / sizeof(ARR[0])) {
+        if (ARR[I] == want) {
+            f(std::integral_constant<int, ARR[I]>{});
+            return true;
+        }
+        return with_occ<ARR, I + 1>(want, f);
+    }
+    return false;
+}
+
 template <class KL>
-int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stages) {
+int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stages, const LaunchCfg &cfg) {
     hipError_t e = hipSuccess;
     if (stages & 1) {
         // lanes of a hashing wave take messages of similar length
@@ -370,8 +405,27 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             // attempt 0 in two kernels tuned for their register needs, then
             // the rank failures (about 1 in 4,000) through the full loop
             const int fail_first = (stages >> 8) & 0xFF;
-            wv::k_sign_prep<TableSign, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
-            wv::k_sign_solve<TableSign, KL::PLANAR><<<(unsigned)n, 64, 0, stream>>>(acc, n, fail_first);
+            const int ordered = cfg.sign_order && n > 64;
+            if (ordered) k_key_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
+            const unsigned grid = (unsigned)n;
+            if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
+                // prepared keys: the variants the launch autotuner chooses among
+                const bool found =
+                    with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto O) {
+                        wv::k_sign_prep<TableSign, KL, decltype(O)::value><<<grid, 64, 0, stream>>>(acc, n, ordered);
+                    }) &&
+                    with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto O) {
+                        wv::k_sign_solve<TableSign, KL::PLANAR, decltype(O)::value>
+                            <<<grid, 64, 0, stream>>>(acc, n, fail_first, ordered);
+                    });
+                if (!found) {
+                    note(hipErrorInvalidValue);
+                    return (int)hipErrorInvalidValue;
+                }
+            } else {
+                wv::k_sign_prep<TableSign, KL><<<grid, 64, 0, stream>>>(acc, n, ordered);
+                wv::k_sign_solve<TableSign, KL::PLANAR><<<grid, 64, 0, stream>>>(acc, n, fail_first, ordered);
+            }
             const unsigned retry_grid = (unsigned)(n < 256 ? n : 256);
             wv::k_sign_retry<TableSign, KL><<<retry_grid, 64 * wv::RNW, 0, stream>>>(acc, n, fail_first);
         }
@@ -382,7 +436,7 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
 }
 
 template <class KL>
-int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int stages) {
+int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int stages, const LaunchCfg &cfg) {
     hipError_t e = hipSuccess;
     if (stages & 1) {
         if (n > 64) {
@@ -394,10 +448,23 @@ int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int 
         e = hipGetLastError();
     }
     if (e == hipSuccess && (stages & 2)) {
-        if (stages & 4)
+        if (stages & 4) {
             k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
-        else
-            wv::k_verify_wave<TableVerify, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+        } else {
+            const int ordered = cfg.verify_order && n > 64;
+            if (ordered) k_key_order<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
+            if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
+                if (!with_occ<wv::VERIFY_OCCS>(cfg.verify, [&](auto O) {
+                        wv::k_verify_wave<TableVerify, KL, decltype(O)::value>
+                            <<<(unsigned)n, 64, 0, stream>>>(acc, n, ordered);
+                    })) {
+                    note(hipErrorInvalidValue);
+                    return (int)hipErrorInvalidValue;
+                }
+            } else {
+                wv::k_verify_wave<TableVerify, KL><<<(unsigned)n, 64, 0, stream>>>(acc, n, ordered);
+            }
+        }
         e = hipGetLastError();
     }
     note(e);
@@ -429,7 +496,7 @@ int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const 
     if (!esk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !workspace)
         return (int)hipErrorInvalidValue;
     TableSign acc{esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, workspace};
-    return sign_table_impl<wv::RefKeys>(acc, n, stream, stages);
+    return sign_table_impl<wv::RefKeys>(acc, n, stream, stages, current_cfg());
 }
 
 size_t cumayo_prepared_esk_bytes(void) { return wv::PrepKeys::ESK_BYTES; }
@@ -459,7 +526,7 @@ int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const 
         !workspace || (stages & 4))
         return (int)hipErrorInvalidValue;
     TableSign acc{prepared, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, workspace};
-    return sign_table_impl<wv::PrepKeys>(acc, n, stream, stages);
+    return sign_table_impl<wv::PrepKeys>(acc, n, stream, stages, current_cfg());
 }
 
 int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
@@ -472,7 +539,7 @@ int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, cons
         return (int)hipErrorInvalidValue;
     TableVerify acc{prepared, wv::PrepKeys::EPK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, results,
                     workspace};
-    return verify_table_impl<wv::PrepKeys>(acc, n, stream, stages);
+    return verify_table_impl<wv::PrepKeys>(acc, n, stream, stages, current_cfg());
 }
 
 int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
@@ -492,7 +559,7 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
     if (!epk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !results || !workspace)
         return (int)hipErrorInvalidValue;
     TableVerify acc{epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs, results, workspace};
-    return verify_table_impl<wv::RefKeys>(acc, n, stream, stages);
+    return verify_table_impl<wv::RefKeys>(acc, n, stream, stages, current_cfg());
 }
 
 int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
@@ -543,6 +610,226 @@ int cumayo_crypto_sign_open(const uint8_t *prepared, const int32_t *key_idx, con
     le = hipGetLastError();
     note(le);
     return (int)le;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Launch configuration, launch autotuner and captured pipelines (C ABI)
+// ---------------------------------------------------------------------------
+struct cumayo_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+namespace {
+
+// Records launch(s) on a private stream in capture mode into an
+// instantiated graph; null (and the error noted) on failure.
+template <class F>
+cumayo_graph *capture(F &&launch) {
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        note(e);
+        return nullptr;
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const int le = launch(s);
+        const hipError_t ee = hipStreamEndCapture(s, &g);
+        e = le ? (hipError_t)le : ee;
+    }
+    (void)hipStreamDestroy(s);
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess && g) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        note(e != hipSuccess ? e : hipErrorUnknown);
+        return nullptr;
+    }
+    return new cumayo_graph{g, x};
+}
+
+// Least duration in ms of launch() over reps timed runs after one untimed
+// run, between two events on `stream`; *err gets the first error.
+template <class F>
+double time_min_ms(hipStream_t stream, hipEvent_t a, hipEvent_t b, int reps, F &&launch, int *err) {
+    double best = 1e30;
+    for (int r = -1; r < reps && !*err; ++r) {
+        hipError_t he = hipEventRecord(a, stream);
+        const int le = he == hipSuccess ? launch() : (int)he;
+        he = hipEventRecord(b, stream);
+        if (he == hipSuccess) he = hipEventSynchronize(b);
+        float ms = 0;
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, a, b);
+        if (le || he != hipSuccess) {
+            *err = le ? le : (int)he;
+            note((hipError_t)*err);
+            break;
+        }
+        if (r >= 0 && ms < best) best = ms;
+    }
+    return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cumayo_get_launch_config(cumayo_launch_config *out) {
+    if (!out) return;
+    const LaunchCfg c = current_cfg();
+    out->sign_prep = c.prep;
+    out->sign_solve = c.solve;
+    out->sign_order = c.sign_order;
+    out->verify = c.verify;
+    out->verify_order = c.verify_order;
+}
+
+int cumayo_set_launch_config(const cumayo_launch_config *cfg) {
+    if (!cfg || !listed(wv::PREP_OCCS, cfg->sign_prep) || !listed(wv::SOLVE_OCCS, cfg->sign_solve) ||
+        !listed(wv::VERIFY_OCCS, cfg->verify) || (unsigned)cfg->sign_order > 1u ||
+        (unsigned)cfg->verify_order > 1u)
+        return (int)hipErrorInvalidValue;
+    g_prep.store(cfg->sign_prep);
+    g_solve.store(cfg->sign_solve);
+    g_sign_order.store(cfg->sign_order);
+    g_verify.store(cfg->verify);
+    g_verify_order.store(cfg->verify_order);
+    return 0;
+}
+
+int cumayo_launch_variants(int field, int32_t *out, int max) {
+    const int *v = nullptr;
+    int cnt = 0;
+    static const int orders[2] = {0, 1};
+    auto use = [&](const auto &arr) { v = arr, cnt = (int)(sizeof(arr) / sizeof(arr[0])); };
+    switch (field) {
+        case CUMAYO_FIELD_SIGN_PREP: use(wv::PREP_OCCS); break;
+        case CUMAYO_FIELD_SIGN_SOLVE: use(wv::SOLVE_OCCS); break;
+        case CUMAYO_FIELD_VERIFY: use(wv::VERIFY_OCCS); break;
+        case CUMAYO_FIELD_ORDER: use(orders); break;
+        default: return -1;
+    }
+    for (int i = 0; i < cnt && i < max && out; ++i) out[i] = v[i];
+    return cnt;
+}
+
+int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int32_t *key_idx,
+                    const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs,
+                    uint8_t *results, size_t n, uint8_t *sign_ws, uint8_t *verify_ws, hipStream_t stream,
+                    int reps, cumayo_tune_report *out) {
+    const bool do_sign = prep_esk && sign_ws, do_verify = prep_epk && verify_ws && results;
+    if (!n || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !out || (!do_sign && !do_verify) ||
+        ((uintptr_t)prep_esk & 15) || ((uintptr_t)prep_epk & 15))
+        return (int)hipErrorInvalidValue;
+    if (reps < 1) reps = 1;
+    std::memset(out, 0, sizeof *out);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t he = hipEventCreate(&a);
+    if (he == hipSuccess) he = hipEventCreate(&b);
+    if (he != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        note(he);
+        return (int)he;
+    }
+    LaunchCfg cfg = current_cfg();
+    int err = 0, timed = 0;
+    if (do_sign) {
+        TableSign acc{prep_esk, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, sign_ws};
+        err = sign_table_impl<wv::PrepKeys>(acc, n, stream, 1, cfg);  // hashing pass, once
+        double best = 1e30;
+        LaunchCfg bc = cfg;
+        for (int order = 0; order < 2; ++order)
+            for (int p : wv::PREP_OCCS)
+                for (int so : wv::SOLVE_OCCS) {
+                    if (err) break;
+                    LaunchCfg c = cfg;
+                    c.prep = p, c.solve = so, c.sign_order = order;
+                    const double t = time_min_ms(
+                        stream, a, b, reps, [&] { return sign_table_impl<wv::PrepKeys>(acc, n, stream, 2, c); },
+                        &err);
+                    ++timed;
+                    if (p == wv::PREP_OCCS[0] && so == wv::SOLVE_OCCS[0] && order == 0) out->sign_default_ms = t;
+                    if (t < best) best = t, bc = c;
+                }
+        cfg.prep = bc.prep, cfg.solve = bc.solve, cfg.sign_order = bc.sign_order;
+        out->sign_ms = best;
+    }
+    if (do_verify && !err) {
+        TableVerify acc{prep_epk, wv::PrepKeys::EPK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, results,
+                        verify_ws};
+        err = verify_table_impl<wv::PrepKeys>(acc, n, stream, 1, cfg);
+        double best = 1e30;
+        LaunchCfg bc = cfg;
+        for (int order = 0; order < 2; ++order)
+            for (int vo : wv::VERIFY_OCCS) {
+                if (err) break;
+                LaunchCfg c = cfg;
+                c.verify = vo, c.verify_order = order;
+                const double t = time_min_ms(
+                    stream, a, b, reps, [&] { return verify_table_impl<wv::PrepKeys>(acc, n, stream, 2, c); },
+                    &err);
+                ++timed;
+                if (vo == wv::VERIFY_OCCS[0] && order == 0) out->verify_default_ms = t;
+                if (t < best) best = t, bc = c;
+            }
+        cfg.verify = bc.verify, cfg.verify_order = bc.verify_order;
+        out->verify_ms = best;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!err) {
+        g_prep.store(cfg.prep);
+        g_solve.store(cfg.solve);
+        g_sign_order.store(cfg.sign_order);
+        g_verify.store(cfg.verify);
+        g_verify_order.store(cfg.verify_order);
+    }
+    cumayo_get_launch_config(&out->best);
+    out->configs_timed = timed;
+    return err;
+}
+
+cumayo_graph *cumayo_sign_graph_create(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                                       const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                                       uint8_t *workspace, int stages) {
+    if (!n) {
+        note(hipErrorInvalidValue);
+        return nullptr;
+    }
+    return capture([&](hipStream_t s) {
+        return cumayo_sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, workspace, s, stages);
+    });
+}
+
+cumayo_graph *cumayo_verify_graph_create(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                                         const int64_t *msg_off, const int32_t *msg_len, const uint8_t *sigs,
+                                         uint8_t *results, size_t n, uint8_t *workspace, int stages) {
+    if (!n) {
+        note(hipErrorInvalidValue);
+        return nullptr;
+    }
+    return capture([&](hipStream_t s) {
+        return cumayo_verify_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, results, n, workspace, s,
+                                      stages);
+    });
+}
+
+int cumayo_graph_launch(cumayo_graph *g, hipStream_t stream) {
+    if (!g) return (int)hipErrorInvalidValue;
+    const hipError_t e = hipGraphLaunch(g->exec, stream);
+    note(e);
+    return (int)e;
+}
+
+void cumayo_graph_destroy(cumayo_graph *g) {
+    if (!g) return;
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    delete g;
 }
 
 }  // extern "C"

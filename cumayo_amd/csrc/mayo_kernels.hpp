@@ -97,6 +97,7 @@ struct TableSign {
     }
     __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)SIGN_SCRATCH_STRIDE; }
     __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
+    __device__ uint32_t key(size_t i) const { return (uint32_t)key_idx[i]; }
     __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + SCR_ORDER) = i; }
     __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + SCR_ORDER); }
 };
@@ -118,6 +119,7 @@ struct PitchedVerify {
         uint8_t *row = pool + i * pitch;
         return {row, row + msg_off, mlen, row + sig_off, row + res_off, nullptr};
     }
+    __device__ size_t order(size_t k) const { return k; }  // pools run in row order
 };
 
 struct TableVerify {
@@ -137,6 +139,7 @@ struct TableVerify {
     }
     __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)VERIFY_SCRATCH_STRIDE; }
     __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
+    __device__ uint32_t key(size_t i) const { return (uint32_t)key_idx[i]; }
     __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + VSCR_ORDER) = i; }
     __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + VSCR_ORDER); }
 };
@@ -222,6 +225,72 @@ __global__ void __launch_bounds__(1024) k_length_order(Acc acc, size_t n) {
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (bin[u] < ORDER_BINS) acc.set_order(k[u], (uint32_t)(i0 + 1024 * u));
+    }
+}
+
+// XCD-aware order of the one-wave-per-item kernels (the slot of the hashing
+// order is reused: the hashing pass has consumed it). The dispatcher deals
+// workgroup b of a launch to XCD b mod XCDS, and each XCD has an L2 of its
+// own. Items are sorted by key (counting sort over the top 8 bits of the
+// key range) and the sorted list is cut into XCDS runs, run x going to the
+// workgroups x, x + XCDS, x + 2 XCDS, ... in order: an XCD then works
+// through its own few keys one after another instead of sharing every key
+// with the other seven. Entry b goes to acc.set_order(b); the result is a
+// permutation of [0, n).
+constexpr int XCDS = 8;
+template <class Acc>
+__global__ void __launch_bounds__(1024) k_key_order(Acc acc, size_t n) {
+    constexpr int U = 8;
+    __shared__ uint32_t pos[ORDER_BINS];
+    __shared__ uint32_t kmax;
+    if (threadIdx.x == 0) kmax = 0;
+    for (int b = threadIdx.x; b < ORDER_BINS; b += 1024) pos[b] = 0;
+    __syncthreads();
+    uint32_t km = 0;
+    for (size_t i = threadIdx.x; i < n; i += 1024) km = max(km, acc.key(i));
+    atomicMax(&kmax, km);
+    __syncthreads();
+    const int bits = kmax ? 32 - __builtin_clz(kmax) : 0;
+    const int sh = bits > 8 ? bits - 8 : 0;  // key >> sh < ORDER_BINS
+    for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
+        uint32_t bin[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bin[u] = i0 + 1024 * u < n ? acc.key(i0 + 1024 * u) >> sh : ORDER_BINS;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (bin[u] < ORDER_BINS) atomicAdd(&pos[bin[u]], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int b = 0; b < ORDER_BINS; ++b) {
+            const uint32_t c = pos[b];
+            pos[b] = sum;
+            sum += c;
+        }
+    }
+    __syncthreads();
+    // run x holds sorted positions [x q + min(x, r), ...) of q + (x < r) items
+    const size_t q = n / XCDS, r = n % XCDS, big = r * (q + 1);
+    for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
+        uint32_t bin[U], k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bin[u] = i0 + 1024 * u < n ? acc.key(i0 + 1024 * u) >> sh : ORDER_BINS;
+#pragma unroll
+        for (int u = 0; u < U; ++u) k[u] = bin[u] < ORDER_BINS ? atomicAdd(&pos[bin[u]], 1u) : 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (bin[u] >= ORDER_BINS) continue;
+            size_t x, s;
+            if (k[u] < big) {
+                x = k[u] / (q + 1);
+                s = k[u] - x * (q + 1);
+            } else {
+                x = r + (k[u] - big) / q;
+                s = (k[u] - big) - (x - r) * q;
+            }
+            acc.set_order(XCDS * s + x, (uint32_t)(i0 + 1024 * u));
+        }
     }
 }
 

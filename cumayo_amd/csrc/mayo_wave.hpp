@@ -48,6 +48,25 @@ constexpr int QF_RT = MAYO_QF_RT ? MAYO_QF_RT : 2;
 constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : 3);
 // waves per SIMD of the solve kernel (the bitsliced columns of [A | y'])
 constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
+// Launch variants of the batch kernels over prepared keys that the launch
+// autotuner chooses among (entry 0 is the default above). Quadratic-form
+// kernels: code 10 x rows per lane + waves per SIMD (23 = two rows per lane
+// per chunk, built for three waves per SIMD); one row halves the bit-plane
+// accumulators (more waves) but shares each column's scalar masks over half
+// as many rows. Solve kernel: waves per SIMD. Every variant computes the
+// same bytes.
+constexpr int qf_code(int rt, int occ) { return 10 * rt + occ; }
+inline constexpr int PREP_OCCS[2] = {qf_code(QF_RT, QF_OCC), qf_code(1, K >= 12 ? 3 : QF_OCC + 1)};
+inline constexpr int SOLVE_OCCS[3] = {SOLVE_OCC, M <= 80 ? 6 : 2, 4};
+inline constexpr int VERIFY_OCCS[3] = {qf_code(QF_RT, QF_OCC), qf_code(QF_RT, QF_OCC - 1),
+                                       qf_code(1, K >= 12 ? 3 : QF_OCC + 1)};
+
+// Item of this workgroup: blockIdx.x, or its entry in the XCD-aware order
+// (k_key_order) when `ordered` is set.
+template <class Acc>
+__device__ __forceinline__ size_t item_of(const Acc &acc, int ordered) {
+    return ordered ? acc.order(blockIdx.x) : blockIdx.x;
+}
 // column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
 // go to the last set so that only one set stays active for most pivots
 constexpr int SHIFT = NCOL - 64 * (NCL - 1);
@@ -229,20 +248,20 @@ struct EntryLoad {
 // form (planar for KL::PLANAR) throughout; Ulds is returned nibble-packed,
 // Mv in the key's form.
 //
-// Lanes = (word w, row slot); a chunk gives each slot QF_RT rows, rows in
+// Lanes = (word w, row slot); a chunk gives each slot RT rows, rows in
 // ascending start column so that every row of a chunk starts within a few
 // columns of the first (those columns are masked per lane).
-template <int NLR, bool FULL>
+template <int NLR, bool FULL, int RT = QF_RT>
 constexpr int qf_chunks() {
-    return (NLR + (FULL ? N : V) + QF_RT * SL - 1) / (QF_RT * SL);
+    return (NLR + (FULL ? N : V) + RT * SL - 1) / (RT * SL);
 }
-template <class KL, int NLR, bool FULL, int NW = 1>
+template <class KL, int NLR, bool FULL, int NW = 1, int RT = QF_RT>
 __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *scol, uint32_t *Ulds,
                                           uint32_t *Mv) {
     constexpr int NP = FULL ? N : V;          // P rows and columns
     constexpr int NVR = NLR + NP;             // rows of the schedule
-    constexpr int RPC = QF_RT * SL;           // rows per chunk
-    constexpr int NCH = qf_chunks<NLR, FULL>();
+    constexpr int RPC = RT * SL;           // rows per chunk
+    constexpr int NCH = qf_chunks<NLR, FULL, RT>();
     constexpr int EB = KL::EB;
     constexpr uint32_t P1o = FULL ? KL::EPK_P1 : KL::ESK_P1;
     // NW waves of the workgroup share the chunks (chunk ch goes to wave ch % NW)
@@ -260,11 +279,11 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         const int v0 = ch * RPC;
         const int vlast = (v0 + RPC < NVR ? v0 + RPC : NVR) - 1;
         const int c_lo = start_of(v0), c_free = start_of(vlast);
-        int st[QF_RT];
-        uint32_t off[QF_RT], stride[QF_RT];
-        uint32_t Tb[QF_RT][K][4];
+        int st[RT];
+        uint32_t off[RT], stride[RT];
+        uint32_t Tb[RT][K][4];
 #pragma unroll
-        for (int tt = 0; tt < QF_RT; ++tt) {
+        for (int tt = 0; tt < RT; ++tt) {
             const int vr = v0 + tt * SL + slot;
             const bool valid = active && vr < NVR;
             st[tt] = valid ? start_of(vr) : NP;
@@ -290,28 +309,28 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         }
         // one column: every row of the chunk takes its entry at c, fetched
         // during the previous column; off[tt] is the next column to fetch
-        EntryLoad<KL> ld[QF_RT];
+        EntryLoad<KL> ld[RT];
         auto fetch = [&]() {
 #pragma unroll
-            for (int tt = 0; tt < QF_RT; ++tt) {
+            for (int tt = 0; tt < RT; ++tt) {
                 ld[tt].fetch(key, off[tt], nv);
                 off[tt] += stride[tt];
             }
         };
         // the rows of slot group tt start at column tt_start[tt] or later
         // (uniform): left of the last group's start fewer groups are summed
-        int tt_start[QF_RT];
+        int tt_start[RT];
 #pragma unroll
-        for (int tt = 0; tt < QF_RT; ++tt) tt_start[tt] = v0 + tt * SL < NVR ? start_of(v0 + tt * SL) : NP;
+        for (int tt = 0; tt < RT; ++tt) tt_start[tt] = v0 + tt * SL < NVR ? start_of(v0 + tt * SL) : NP;
         // groups 0 .. NTT-1 take column c
         auto column = [&](auto NT, int c, bool masked, bool more) {
             constexpr int NTT = decltype(NT)::value;
             const uint64_t sc = uni64(scol[c]);
             // masks from the 32-bit halves: one scalar bit-field extract each
             const uint32_t sh[2] = {(uint32_t)sc, (uint32_t)(sc >> 32)};
-            uint32_t x[QF_RT];
+            uint32_t x[RT];
 #pragma unroll
-            for (int tt = 0; tt < QF_RT; ++tt) {
+            for (int tt = 0; tt < RT; ++tt) {
                 x[tt] = ld[tt].word(nv);
                 if (masked) x[tt] = c >= st[tt] ? x[tt] : 0u;
             }
@@ -326,13 +345,13 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                     for (int tt = 0; tt < NTT; ++tt) Tb[tt][j][b] = gf16::axm(Tb[tt][j][b], x[tt], m);
                 }
         };
-        using All = std::integral_constant<int, QF_RT>;
+        using All = std::integral_constant<int, RT>;
         // columns [c0, cend), masked below cm, in three loops: only group 0
-        // started (QF_RT = 2), all groups masked, unmasked
+        // started (RT = 2), all groups masked, unmasked
         auto segment = [&](int c0, int cm, int cend) {
             if (c0 < cend) fetch();
             int c = c0;
-            if constexpr (QF_RT == 2) {
+            if constexpr (RT == 2) {
                 const int c1 = tt_start[1] < cm ? tt_start[1] : cm;
 #pragma unroll 1
                 for (; c < c1; ++c) column(std::integral_constant<int, 1>{}, c, true, c + 1 < cend);
@@ -348,7 +367,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
             // columns >= V: P2 for rows < V, P3 for rows >= V
             const int cb = c_lo > V ? c_lo : V;
 #pragma unroll
-            for (int tt = 0; tt < QF_RT; ++tt) {
+            for (int tt = 0; tt < RT; ++tt) {
                 const int r = v0 + tt * SL + slot;
                 if (active && r < NP) {
                     if (r < V) {
@@ -365,7 +384,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         }
         // rows done: T_j = sum_b a^b T_jb
 #pragma unroll
-        for (int tt = 0; tt < QF_RT; ++tt) {
+        for (int tt = 0; tt < RT; ++tt) {
             const int vr = v0 + tt * SL + slot;
             uint32_t T[K];
 #pragma unroll
@@ -998,20 +1017,20 @@ __global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, i
 
 // Batch form, attempt 0, part 1: M, U and y, handed over through the
 // workspace. Register-heavy (the pair accumulators of the quadratic form).
-template <class Acc, class KL = RefKeys>
-__global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
+template <class Acc, class KL = RefKeys, int CODE = qf_code(QF_RT, QF_OCC)>
+__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
-    SignItem it = acc(blockIdx.x);
+    SignItem it = acc(item_of(acc, ordered));
     // the failure list of the solve kernel starts empty
-    if (blockIdx.x == 0 && lane == 0) *(uint32_t *)(it.scratch + SCR_LIST_COUNT) = 0;
+    if (blockIdx.x == 0 && lane == 0) *(uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT) = 0;
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     __syncthreads();
     wave_scol(S);
     for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
     __syncthreads();
-    quad_form<KL, O, false>(it.esk, S.scol, S.U, S.Mv);
+    quad_form<KL, O, false, 1, CODE / 10>(it.esk, S.scol, S.U, S.Mv);
     __syncthreads();
     wave_y(S, S.t);
     uint32_t *mv = (uint32_t *)(it.scratch + SCR_MV);
@@ -1023,11 +1042,12 @@ __global__ void __launch_bounds__(64, QF_OCC) k_sign_prep(Acc acc, size_t n) {
 // Batch form, attempt 0, part 2: columns of [A | y - A r], Gauss-Jordan,
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
 // PL: the M vectors in the workspace are planar (prepared keys).
-template <class Acc, bool PL>
-__global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0) {
+template <class Acc, bool PL, int OCC = SOLVE_OCC>
+__global__ void __launch_bounds__(64, OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0, int ordered = 0) {
     __shared__ SolveShared S;
     const int lane = threadIdx.x;
-    SignItem it = acc(blockIdx.x);
+    const size_t item = item_of(acc, ordered);
+    SignItem it = acc(item);
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     planes_of_M<PL>((const uint32_t *)(it.scratch + SCR_MV), S.Mb);
     const uint32_t *yv = (const uint32_t *)(it.scratch + SCR_Y);
@@ -1042,7 +1062,7 @@ __global__ void __launch_bounds__(64, SOLVE_OCC) k_sign_solve(Acc acc, size_t n,
         if (lane == 0) {
             it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
             const uint32_t k = atomicAdd((uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT), 1u);
-            *(uint32_t *)(acc.scratch_of(k) + SCR_LIST) = (uint32_t)blockIdx.x;
+            *(uint32_t *)(acc.scratch_of(k) + SCR_LIST) = (uint32_t)item;
         }
         return;
     }
@@ -1226,15 +1246,15 @@ __global__ void __launch_bounds__(64, 2) k_keypair_P3_wave(uint8_t *pool, size_t
 }
 
 // Verification, one wave per item (verify.cu:9-211).
-template <class Acc, class KL = RefKeys>
-__global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
+template <class Acc, class KL = RefKeys, int CODE = qf_code(QF_RT, QF_OCC)>
+__global__ void __launch_bounds__(64, CODE % 10) k_verify_wave(Acc acc, size_t n, int ordered = 0) {
     __shared__ uint64_t scol[N];
     __shared__ uint32_t U[PAIRS * MW];
     __shared__ uint32_t yu[YG_MAX * UW];
     __shared__ uint8_t tb[MB + 4];
     __shared__ uint8_t hbuf[DIGEST + SK];
     const int lane = threadIdx.x;
-    VerifyItem it = acc(blockIdx.x);
+    VerifyItem it = acc(item_of(acc, ordered));
     for (int c = lane; c < N; c += 64) {
         uint64_t v = 0;
         for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(it.sig, j * N + c) << (4 * j);
@@ -1251,7 +1271,7 @@ __global__ void __launch_bounds__(64, QF_OCC) k_verify_wave(Acc acc, size_t n) {
         keccak::wave_shake256(tb, MB, hbuf, DIGEST + SK);
     }
     __syncthreads();
-    quad_form<KL, 0, true>(it.epk, scol, U, nullptr);
+    quad_form<KL, 0, true, 1, CODE / 10>(it.epk, scol, U, nullptr);
     __syncthreads();
     unreduced_y<BlockSync>(U, yu);
     if (lane == 0) {

@@ -68,6 +68,22 @@ class Sizes:
         return self.k * self.m * ((self.o + 1) // 2)
 
 
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [("sign_prep", ctypes.c_int32), ("sign_solve", ctypes.c_int32), ("sign_order", ctypes.c_int32),
+                ("verify", ctypes.c_int32), ("verify_order", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f[0]: int(getattr(self, f[0])) for f in self._fields_}
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("best", LaunchConfig), ("sign_ms", ctypes.c_double), ("verify_ms", ctypes.c_double),
+                ("sign_default_ms", ctypes.c_double), ("verify_default_ms", ctypes.c_double),
+                ("configs_timed", ctypes.c_int32)]
+
+
+LAUNCH_FIELDS = {"sign_prep": 0, "sign_solve": 1, "verify": 2, "order": 3}
+
 # (name, argtypes) of every symbol the header declares
 _VP = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -111,6 +127,15 @@ EXPORTS = {
     "cumayo_crypto_sign": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
     "cumayo_crypto_sign_open_workspace_size": [_SZ],
     "cumayo_crypto_sign_open": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
+    "cumayo_get_launch_config": [ctypes.POINTER(LaunchConfig)],
+    "cumayo_set_launch_config": [ctypes.POINTER(LaunchConfig)],
+    "cumayo_launch_variants": [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int],
+    "cumayo_autotune": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, _VP, ctypes.c_int,
+                        ctypes.POINTER(TuneReport)],
+    "cumayo_sign_graph_create": [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, ctypes.c_int],
+    "cumayo_verify_graph_create": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, ctypes.c_int],
+    "cumayo_graph_launch": [_VP, _VP],
+    "cumayo_graph_destroy": [_VP],
 }
 _RESTYPES = {
     "cumayo_last_error": ctypes.c_int,
@@ -130,6 +155,12 @@ _RESTYPES = {
     "cumayo_crypto_sign": ctypes.c_int,
     "cumayo_crypto_sign_open_workspace_size": _SZ,
     "cumayo_crypto_sign_open": ctypes.c_int,
+    "cumayo_set_launch_config": ctypes.c_int,
+    "cumayo_launch_variants": ctypes.c_int,
+    "cumayo_autotune": ctypes.c_int,
+    "cumayo_sign_graph_create": _VP,
+    "cumayo_verify_graph_create": _VP,
+    "cumayo_graph_launch": ctypes.c_int,
 }
 
 
@@ -473,3 +504,83 @@ class Engine:
         if err:
             raise RuntimeError(f"cumayo_crypto_sign_open failed: HIP error {err}")
         return ws
+
+    # -- launch autotuner and captured pipelines ---------------------------
+    def launch_config(self) -> dict:
+        c = LaunchConfig()
+        self.lib.cumayo_get_launch_config(ctypes.byref(c))
+        return c.as_dict()
+
+    def set_launch_config(self, **fields):
+        c = LaunchConfig(**{**self.launch_config(), **fields})
+        err = self.lib.cumayo_set_launch_config(ctypes.byref(c))
+        if err:
+            raise ValueError(f"not a compiled launch variant: {c.as_dict()}")
+
+    def launch_variants(self) -> dict:
+        out = {}
+        for name, f in LAUNCH_FIELDS.items():
+            buf = (ctypes.c_int32 * 8)()
+            cnt = self.lib.cumayo_launch_variants(f, buf, 8)
+            out[name] = [int(buf[i]) for i in range(cnt)]
+        return out
+
+    def autotune(self, prep_esk, prep_epk, key_idx, msgs, msg_off, msg_len, sigs, results=None,
+                 sign_ws=None, verify_ws=None, reps: int = 3, stream=None) -> dict:
+        """Times the linear-algebra pass under every compiled launch variant
+        on this batch and keeps the fastest (cumayo_autotune)."""
+        n = int(key_idx.numel())
+        rep = TuneReport()
+        err = self.lib.cumayo_autotune(
+            _ptr(prep_esk), _ptr(prep_epk), key_idx.data_ptr(), msgs.data_ptr(), msg_off.data_ptr(),
+            msg_len.data_ptr(), sigs.data_ptr(), _ptr(results), n, _ptr(sign_ws), _ptr(verify_ws),
+            self.stream_handle(stream), reps, ctypes.byref(rep))
+        if err:
+            raise RuntimeError(f"cumayo_autotune failed: HIP error {err}")
+        return {"best": rep.best.as_dict(), "sign_ms": rep.sign_ms, "verify_ms": rep.verify_ms,
+                "sign_default_ms": rep.sign_default_ms, "verify_default_ms": rep.verify_default_ms,
+                "configs_timed": int(rep.configs_timed)}
+
+    def sign_graph(self, prep_esk, key_idx, msgs, msg_off, msg_len, sigs, workspace, stages=3) -> "Graph":
+        """cumayo_sign_prepared over these buffers, captured as one HIP graph."""
+        h = self.lib.cumayo_sign_graph_create(prep_esk.data_ptr(), key_idx.data_ptr(), msgs.data_ptr(),
+                                              msg_off.data_ptr(), msg_len.data_ptr(), sigs.data_ptr(),
+                                              int(key_idx.numel()), workspace.data_ptr(), stages)
+        if not h:
+            raise RuntimeError(f"cumayo_sign_graph_create failed: HIP error {self.lib.cumayo_last_error()}")
+        return Graph(self, h)
+
+    def verify_graph(self, prep_epk, key_idx, msgs, msg_off, msg_len, sigs, results, workspace,
+                     stages=3) -> "Graph":
+        h = self.lib.cumayo_verify_graph_create(prep_epk.data_ptr(), key_idx.data_ptr(), msgs.data_ptr(),
+                                                msg_off.data_ptr(), msg_len.data_ptr(), sigs.data_ptr(),
+                                                results.data_ptr(), int(key_idx.numel()), workspace.data_ptr(),
+                                                stages)
+        if not h:
+            raise RuntimeError(f"cumayo_verify_graph_create failed: HIP error {self.lib.cumayo_last_error()}")
+        return Graph(self, h)
+
+
+class Graph:
+    """A captured batch pipeline; launch() replays it on a stream."""
+
+    def __init__(self, engine: Engine, handle: int):
+        self.engine, self.handle = engine, handle
+
+    def launch(self, stream=None):
+        if not self.handle:
+            raise RuntimeError("graph destroyed")
+        err = self.engine.lib.cumayo_graph_launch(self.handle, self.engine.stream_handle(stream))
+        if err:
+            raise RuntimeError(f"cumayo_graph_launch failed: HIP error {err}")
+
+    def close(self):
+        if self.handle:
+            self.engine.lib.cumayo_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

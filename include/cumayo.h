@@ -13,9 +13,10 @@
  * cumayo_last_error().
  *
  * Part 2 extends the boundary with the batched key-table form used for
- * shared resident keys and ragged messages, and Part 3 with a
- * continuous-batching request server over it; neither has a counterpart in
- * the reference.
+ * shared resident keys and ragged messages, Part 3 with a
+ * continuous-batching request server over it, and Part 4 with the launch
+ * autotuner and HIP-graph captures of the batch pipelines; none of them has
+ * a counterpart in the reference.
  */
 #ifndef CUMAYO_H
 #define CUMAYO_H
@@ -284,6 +285,79 @@ int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
                              const int64_t *msg_off, const int32_t *msg_len, const int32_t *msg_key,
                              const uint8_t *sig_pool, size_t pool_n, cumayo_serve_report *out,
                              int64_t *lat_out, size_t lat_cap);
+
+/* ---------------- Part 4: launch autotuner and captured pipelines ---------------- */
+
+/* Launch configuration of the batch entry points (process-wide for this
+ * library; read at every launch and at graph capture). Over prepared key
+ * tables the wave kernels come in compiled variants: the quadratic-form
+ * kernels (sign_prep, verify) are coded 10 x rows per lane + waves per SIMD
+ * they are built for (23: each lane owns two rows of a chunk, three waves per
+ * SIMD; one row per lane halves the accumulators but shares each column's
+ * scalar masks over fewer rows), the solve kernel by its waves per SIMD; the
+ * one-wave-per-item kernels of every table form run their items either in
+ * batch order (0) or in an XCD-aware order (1: items sorted by key and dealt
+ * to the eight XCDs in contiguous runs, so each XCD's L2 holds few keys at a
+ * time; one extra one-workgroup sort per launch, batches over 64 items
+ * only). Every configuration gives the same bytes. */
+typedef struct {
+    int32_t sign_prep;    /* quadratic form + M + y kernel (10 x rows per lane + waves per SIMD) */
+    int32_t sign_solve;   /* columns of A + elimination kernel (waves per SIMD) */
+    int32_t sign_order;   /* 0 batch order, 1 XCD-aware key order */
+    int32_t verify;       /* verification kernel (10 x rows per lane + waves per SIMD) */
+    int32_t verify_order;
+} cumayo_launch_config;
+
+#define CUMAYO_FIELD_SIGN_PREP 0
+#define CUMAYO_FIELD_SIGN_SOLVE 1
+#define CUMAYO_FIELD_VERIFY 2
+#define CUMAYO_FIELD_ORDER 3
+
+void cumayo_get_launch_config(cumayo_launch_config *out);
+/* Returns hipErrorInvalidValue (nothing changed) for a value that is not a
+ * compiled variant. */
+int cumayo_set_launch_config(const cumayo_launch_config *cfg);
+/* The values a field can take (entry 0 is the default); writes up to max of
+ * them to out and returns their number, or -1 for an unknown field. */
+int cumayo_launch_variants(int field, int32_t *out, int max);
+
+typedef struct {
+    cumayo_launch_config best;          /* the configuration now in force */
+    double sign_ms, verify_ms;          /* linear-algebra pass, best configuration */
+    double sign_default_ms, verify_default_ms;  /* the same with the defaults */
+    int32_t configs_timed;
+} cumayo_tune_report;
+
+/* Launch autotuner: times the linear-algebra pass (stage 2) of
+ * cumayo_sign_prepared and cumayo_verify_prepared on the caller's own batch
+ * under every combination of compiled variants and item orders (HIP events
+ * on `stream`, the least of `reps` runs after an untimed one), then makes the
+ * fastest the configuration in force. The hashing pass runs once first.
+ * Signing is tuned when prep_esk and sign_ws are given, verification when
+ * prep_epk, verify_ws and results are; verification reads sigs. sigs,
+ * results and the workspaces end up holding the results of the batch.
+ * Synchronises `stream`. Returns 0 or a hipError_t. */
+int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int32_t *key_idx,
+                    const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs,
+                    uint8_t *results, size_t n, uint8_t *sign_ws, uint8_t *verify_ws, hipStream_t stream,
+                    int reps, cumayo_tune_report *out);
+
+/* HIP-graph captures of cumayo_sign_prepared / cumayo_verify_prepared for
+ * fixed buffers and batch size (the kernels of both stages, under the launch
+ * configuration in force at capture; no memory set or copy nodes). Each
+ * cumayo_graph_launch replays the whole pipeline on `stream` as one graph
+ * launch, with the inputs' contents at that time. Create returns NULL on
+ * invalid arguments or a capture error (cumayo_last_error). Replays of one
+ * graph must not overlap each other or other work on the same buffers. */
+typedef struct cumayo_graph cumayo_graph;
+cumayo_graph *cumayo_sign_graph_create(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                                       const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                                       uint8_t *workspace, int stages);
+cumayo_graph *cumayo_verify_graph_create(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                                         const int64_t *msg_off, const int32_t *msg_len, const uint8_t *sigs,
+                                         uint8_t *results, size_t n, uint8_t *workspace, int stages);
+int cumayo_graph_launch(cumayo_graph *g, hipStream_t stream);
+void cumayo_graph_destroy(cumayo_graph *g);
 
 #ifdef __cplusplus
 }

@@ -59,3 +59,42 @@ def test_library_exports(version):
     s = _Sizes()
     lib.cumayo_get_sizes(ctypes.byref(s))
     assert s.version == version
+
+
+@pytest.mark.parametrize("version", [1, 2, 3, 5])
+def test_launch_variants_and_config(version):
+    """The launch autotuner's variant lists and configuration, without a
+    GPU: entry 0 of every list is the configuration in force at load, a
+    compiled variant is accepted and an unknown one refused unchanged."""
+    from cumayo_amd import build
+    from cumayo_amd.engine import EXPORTS, LAUNCH_FIELDS, LaunchConfig, load_library
+    if not os.path.exists(build.lib_path(version)):
+        pytest.skip("library not built")
+    lib = load_library(version)
+    variants = {}
+    for name, f in LAUNCH_FIELDS.items():
+        buf = (ctypes.c_int32 * 8)()
+        cnt = lib.cumayo_launch_variants(f, buf, 8)
+        assert 2 <= cnt <= 8, name
+        variants[name] = [buf[i] for i in range(cnt)]
+        assert len(set(variants[name])) == cnt, (name, variants[name])
+    assert lib.cumayo_launch_variants(99, None, 0) == -1
+    assert variants["order"] == [0, 1]
+    cfg = LaunchConfig()
+    lib.cumayo_get_launch_config(ctypes.byref(cfg))
+    assert cfg.as_dict() == {"sign_prep": variants["sign_prep"][0], "sign_solve": variants["sign_solve"][0],
+                             "sign_order": 0, "verify": variants["verify"][0], "verify_order": 0}
+    # quadratic-form variants are coded 10 x rows per lane + waves per SIMD
+    for code in variants["sign_prep"] + variants["verify"]:
+        assert code // 10 in (1, 2) and 1 <= code % 10 <= 8, code
+    alt = LaunchConfig(variants["sign_prep"][1], variants["sign_solve"][2], 1, variants["verify"][1], 1)
+    assert lib.cumayo_set_launch_config(ctypes.byref(alt)) == 0
+    got = LaunchConfig()
+    lib.cumayo_get_launch_config(ctypes.byref(got))
+    assert got.as_dict() == alt.as_dict()
+    bad = LaunchConfig(variants["sign_prep"][0] + 50, variants["sign_solve"][0], 0, variants["verify"][0], 0)
+    assert lib.cumayo_set_launch_config(ctypes.byref(bad)) != 0
+    lib.cumayo_get_launch_config(ctypes.byref(got))
+    assert got.as_dict() == alt.as_dict()
+    assert lib.cumayo_set_launch_config(ctypes.byref(cfg)) == 0
+    assert "cumayo_autotune" in EXPORTS

@@ -1,0 +1,172 @@
+"""GPU: the launch autotuner's variants and the HIP-graph captures of the
+batch pipelines. Every launch variant and item order must give the bytes of
+the default launch (which the parity tests pin to the oracle); every replay
+of a captured graph must give the eager result, with eager work and
+synchronisations between replays.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(eng, n, nkeys, seed, maxlen=600):
+    import torch
+    esk, epk = eng.keypair_tables(nkeys, rand_index=31 + seed)
+    pesk, pepk = eng.prepare_tables(esk, epk)
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, maxlen, n).astype(np.int32)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+    kidx = rng.integers(0, nkeys, n).astype(np.int32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(eng.device)
+    return {"esk": esk, "pesk": pesk, "pepk": pepk, "k": t(kidx), "m": t(flat), "o": t(offs), "l": t(lens),
+            "kidx": kidx, "flat": flat, "offs": offs, "lens": lens}
+
+
+@pytest.fixture
+def restore_config(engine_factory):
+    saved = {}
+
+    def keep(eng):
+        saved.setdefault(eng.version, (eng, eng.launch_config()))
+        return eng
+
+    yield keep
+    for eng, cfg in saved.values():
+        eng.set_launch_config(**cfg)
+
+
+@pytest.mark.parametrize("v", (1, 2, 3, 5), ids=lambda v: f"MAYO_{v}")
+def test_every_launch_variant_gives_the_same_bytes(engine_factory, restore_config, v):
+    import torch
+    eng = restore_config(engine_factory(v))
+    z = eng.sizes
+    n = 300
+    b = _batch(eng, n, 5, 40 + v)
+    var = eng.launch_variants()
+    print(f"[tuning] MAYO_{v} variants {var}", flush=True)
+    ws = eng.sign_workspace(n)
+    ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=eng.device)
+    eng.set_launch_config(**{k: var[k][0] for k in ("sign_prep", "sign_solve", "verify")},
+                          sign_order=0, verify_order=0)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], ref, ws)
+    ref_att = eng.sign_attempts(ws, n)
+    bad = ref.clone()
+    bad[::4, 3] ^= 0x11
+    vref = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+    eng.verify_prepared(b["pepk"], b["k"], b["m"], b["o"], b["l"], bad, vref, eng.verify_workspace(n))
+    torch.cuda.synchronize()
+    expect = np.zeros(n, np.uint8)
+    expect[::4] = 1
+    assert np.array_equal(vref.cpu().numpy(), expect)
+    for p, s, order in itertools.product(var["sign_prep"], var["sign_solve"], var["order"]):
+        eng.set_launch_config(sign_prep=p, sign_solve=s, sign_order=order)
+        out = torch.full_like(ref, 0xA5)
+        eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], out, ws)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), (p, s, order)
+        assert np.array_equal(eng.sign_attempts(ws, n), ref_att), (p, s, order)
+    for vv, order in itertools.product(var["verify"], var["order"]):
+        eng.set_launch_config(verify=vv, verify_order=order)
+        res = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+        eng.verify_prepared(b["pepk"], b["k"], b["m"], b["o"], b["l"], bad, res, eng.verify_workspace(n))
+        torch.cuda.synchronize()
+        assert torch.equal(res, vref), (vv, order)
+    # the default launch is the oracle's (spot check)
+    esk = b["esk"].cpu().numpy()
+    r = ref.cpu().numpy()
+    for i in (0, 1, n - 1):
+        m = b["flat"][b["offs"][i]:b["offs"][i] + b["lens"][i]].tobytes()
+        assert r[i].tobytes() == O.sign(v, esk[b["kidx"][i]].tobytes(), m)[0], i
+
+
+def test_autotune_keeps_a_compiled_variant_and_the_results(engine_factory, restore_config):
+    import torch
+    v = 1
+    eng = restore_config(engine_factory(v))
+    z = eng.sizes
+    n = 4096
+    b = _batch(eng, n, 16, 7)
+    ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=eng.device)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], ref, eng.sign_workspace(n))
+    sigs = torch.zeros_like(ref)
+    res = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+    print(f"[tuning] autotune MAYO_{v} n={n}", flush=True)
+    rep = eng.autotune(b["pesk"], b["pepk"], b["k"], b["m"], b["o"], b["l"], sigs, res,
+                       eng.sign_workspace(n), eng.verify_workspace(n), reps=2)
+    torch.cuda.synchronize()
+    print(f"[tuning] {rep}", flush=True)
+    var = eng.launch_variants()
+    best = rep["best"]
+    assert best == eng.launch_config()
+    assert best["sign_prep"] in var["sign_prep"] and best["sign_solve"] in var["sign_solve"]
+    assert best["verify"] in var["verify"] and best["sign_order"] in (0, 1) and best["verify_order"] in (0, 1)
+    assert rep["configs_timed"] == 2 * len(var["sign_prep"]) * len(var["sign_solve"]) + 2 * len(var["verify"])
+    assert 0 < rep["sign_ms"] <= rep["sign_default_ms"] and 0 < rep["verify_ms"] <= rep["verify_default_ms"]
+    assert torch.equal(sigs, ref)
+    assert res.cpu().numpy().tolist() == [0] * n
+    # the tuned configuration signs the same bytes
+    again = torch.zeros_like(ref)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], again, eng.sign_workspace(n))
+    torch.cuda.synchronize()
+    assert torch.equal(again, ref)
+
+
+@pytest.mark.parametrize("forced", (0, 2), ids=("plain", "forced-retries"))
+def test_graph_replays_match_eager(engine_factory, forced):
+    """Sign and verify pipelines captured as HIP graphs, replayed three
+    times with eager work (clearing the outputs) and a synchronisation
+    between replays: every replay gives the eager bytes, also when every
+    item goes through the retry pass (its failure list is reset inside the
+    kernels, not by a memory-set node)."""
+    import torch
+    v = 1
+    eng = engine_factory(v)
+    z = eng.sizes
+    n = 1536
+    b = _batch(eng, n, 6, 3 + forced, maxlen=900)
+    stages = 3 | (forced << 8)
+    ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=eng.device)
+    ws_e = eng.sign_workspace(n)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], ref, ws_e, stages=stages)
+    vref = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+    eng.verify_prepared(b["pepk"], b["k"], b["m"], b["o"], b["l"], ref, vref, eng.verify_workspace(n))
+    torch.cuda.synchronize()
+    att = eng.sign_attempts(ws_e, n)
+    assert (att >= forced + 1).all()
+    assert vref.cpu().numpy().tolist() == [0] * n
+    sigs = torch.zeros_like(ref)
+    ws = eng.sign_workspace(n)
+    res = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+    print(f"[tuning] graphs MAYO_{v} n={n} forced={forced}", flush=True)
+    gs = eng.sign_graph(b["pesk"], b["k"], b["m"], b["o"], b["l"], sigs, ws, stages=stages)
+    gv = eng.verify_graph(b["pepk"], b["k"], b["m"], b["o"], b["l"], sigs, res, eng.verify_workspace(n))
+    try:
+        for rep in range(3):
+            sigs.zero_()
+            res.fill_(7)
+            torch.cuda.synchronize()
+            gs.launch()
+            gv.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(sigs, ref), f"sign replay {rep}"
+            assert np.array_equal(eng.sign_attempts(ws, n), att), f"attempts replay {rep}"
+            assert torch.equal(res, vref), f"verify replay {rep}"
+            # eager work between replays on the same buffers
+            eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], sigs, ws, stages=stages)
+            torch.cuda.synchronize()
+            assert torch.equal(sigs, ref)
+    finally:
+        gs.close()
+        gv.close()
+    esk = b["esk"].cpu().numpy()
+    r = ref.cpu().numpy()
+    for i in (0, n // 2, n - 1):
+        m = b["flat"][b["offs"][i]:b["offs"][i] + b["lens"][i]].tobytes()
+        osig, oatt = O.sign(v, esk[b["kidx"][i]].tobytes(), m, fail_first=forced)
+        assert r[i].tobytes() == osig and int(att[i]) == oatt, i
