@@ -66,6 +66,7 @@ void ensure_tables() {
 }
 
 inline unsigned blocks64(size_t n) { return (unsigned)((n + 63) / 64); }
+inline unsigned order_grid(size_t n) { return (unsigned)((n + ORDER_SEG - 1) / ORDER_SEG); }
 
 // Rows of `width` bytes from src (pitch spitch) to dst (pitch dpitch); one
 // workgroup per group of rows, dwords when every address and pitch allows.
@@ -391,7 +392,7 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
     if (stages & 1) {
         // lanes of a hashing wave take messages of similar length
         if (n > 64) {
-            k_length_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
+            k_length_order<TableSign><<<order_grid(n), 1024, 0, stream>>>(acc, n);
             k_sign_hash<TableSign, true><<<blocks64(n), 64, 0, stream>>>(acc, n);
         } else {
             k_sign_hash<TableSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
@@ -440,7 +441,7 @@ int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int 
     hipError_t e = hipSuccess;
     if (stages & 1) {
         if (n > 64) {
-            k_length_order<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
+            k_length_order<TableVerify><<<order_grid(n), 1024, 0, stream>>>(acc, n);
             k_verify_hash<TableVerify, true><<<blocks64(n), 64, 0, stream>>>(acc, n);
         } else {
             k_verify_hash<TableVerify><<<blocks64(n), 64, 0, stream>>>(acc, n);

@@ -188,44 +188,66 @@ __global__ void __launch_bounds__(64) k_seed(uint8_t *pool, size_t pitch, size_t
     }
 }
 
+// Exclusive prefix sum of the ORDER_BINS counters in pos (LDS), in place, by
+// a 1024-lane workgroup: each of the first ORDER_BINS / 64 waves scans 64
+// counters with lane shuffles, then adds the totals of the waves before it.
+__device__ __forceinline__ void scan_bins(uint32_t *pos, uint32_t *wsum) {
+    static_assert(ORDER_BINS % 64 == 0 && ORDER_BINS <= 1024, "bins");
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t v = 0, incl = 0;
+    if (t < ORDER_BINS) {
+        v = pos[t];
+        incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += u;
+        }
+        if (lane == 63) wsum[w] = incl;
+    }
+    __syncthreads();
+    if (t < ORDER_BINS) {
+        uint32_t before = 0;
+        for (int k = 0; k < w; ++k) before += wsum[k];
+        pos[t] = before + incl - v;
+    }
+    __syncthreads();
+}
+
 // Hashing order for the lane-per-item hashing kernels below: items grouped
 // by the SHAKE256 blocks their message absorbs, so that the lanes of a wave
 // hash messages of similar length (a wave lasts as long as its longest
-// message). Counting sort in one workgroup; entry k goes to acc.set_order(k).
+// message). Workgroup s sorts the segment [s ORDER_SEG, (s+1) ORDER_SEG) of
+// the batch by counting sort into the same positions: a wave's 64 lanes only
+// need neighbours of similar length, and the segments sort in parallel.
+// Entry k goes to acc.set_order(k).
+constexpr int ORDER_U = 4;                  // items per thread
+constexpr size_t ORDER_SEG = 1024 * ORDER_U;
 template <class Acc>
 __global__ void __launch_bounds__(1024) k_length_order(Acc acc, size_t n) {
-    constexpr int U = 8;  // items per thread per step: their loads are in flight together
     __shared__ uint32_t pos[ORDER_BINS];
+    __shared__ uint32_t wsum[ORDER_BINS / 64];
+    const size_t base = (size_t)blockIdx.x * ORDER_SEG;
+    const size_t end = base + ORDER_SEG < n ? base + ORDER_SEG : n;
     for (int b = threadIdx.x; b < ORDER_BINS; b += 1024) pos[b] = 0;
     __syncthreads();
-    for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
-        uint32_t bin[U];
+    uint32_t bin[ORDER_U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) bin[u] = i0 + 1024 * u < n ? acc.bin(i0 + 1024 * u) : ORDER_BINS;
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (bin[u] < ORDER_BINS) atomicAdd(&pos[bin[u]], 1u);
+    for (int u = 0; u < ORDER_U; ++u) {
+        const size_t i = base + threadIdx.x + 1024 * u;
+        bin[u] = i < end ? acc.bin(i) : ORDER_BINS;
     }
+#pragma unroll
+    for (int u = 0; u < ORDER_U; ++u)
+        if (bin[u] < ORDER_BINS) atomicAdd(&pos[bin[u]], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t sum = 0;
-        for (int b = 0; b < ORDER_BINS; ++b) {
-            const uint32_t c = pos[b];
-            pos[b] = sum;
-            sum += c;
-        }
-    }
-    __syncthreads();
-    for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
-        uint32_t bin[U], k[U];
+    scan_bins(pos, wsum);
+    uint32_t k[ORDER_U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) bin[u] = i0 + 1024 * u < n ? acc.bin(i0 + 1024 * u) : ORDER_BINS;
+    for (int u = 0; u < ORDER_U; ++u) k[u] = bin[u] < ORDER_BINS ? atomicAdd(&pos[bin[u]], 1u) : 0u;
 #pragma unroll
-        for (int u = 0; u < U; ++u) k[u] = bin[u] < ORDER_BINS ? atomicAdd(&pos[bin[u]], 1u) : 0u;
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (bin[u] < ORDER_BINS) acc.set_order(k[u], (uint32_t)(i0 + 1024 * u));
-    }
+    for (int u = 0; u < ORDER_U; ++u)
+        if (bin[u] < ORDER_BINS) acc.set_order(base + k[u], (uint32_t)(base + threadIdx.x + 1024 * u));
 }
 
 // XCD-aware order of the one-wave-per-item kernels (the slot of the hashing
@@ -242,6 +264,7 @@ template <class Acc>
 __global__ void __launch_bounds__(1024) k_key_order(Acc acc, size_t n) {
     constexpr int U = 8;
     __shared__ uint32_t pos[ORDER_BINS];
+    __shared__ uint32_t wsum[ORDER_BINS / 64];
     __shared__ uint32_t kmax;
     if (threadIdx.x == 0) kmax = 0;
     for (int b = threadIdx.x; b < ORDER_BINS; b += 1024) pos[b] = 0;
@@ -261,15 +284,7 @@ __global__ void __launch_bounds__(1024) k_key_order(Acc acc, size_t n) {
             if (bin[u] < ORDER_BINS) atomicAdd(&pos[bin[u]], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t sum = 0;
-        for (int b = 0; b < ORDER_BINS; ++b) {
-            const uint32_t c = pos[b];
-            pos[b] = sum;
-            sum += c;
-        }
-    }
-    __syncthreads();
+    scan_bins(pos, wsum);
     // run x holds sorted positions [x q + min(x, r), ...) of q + (x < r) items
     const size_t q = n / XCDS, r = n % XCDS, big = r * (q + 1);
     for (size_t i0 = threadIdx.x; i0 < n; i0 += 1024 * U) {
