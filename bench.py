@@ -144,6 +144,58 @@ def mixed_sets_run(eng1, args, weights=None, pool=8192, keys=64, loads=(0.25, 0.
             "vs_load": points}
 
 
+def mixed_batch_run(eng1, n, steps, weights=None, keys=64):
+    """One batch mixing the four parameter sets: item j belongs to MAYO_1/2/3/5
+    with probabilities 40/20/20/20 (from its own SHAKE stream), key j mod 64
+    of its set, message lengths uniform in [32, 1024] B. Sign and verify
+    passes over the whole batch with every set's part on its own stream,
+    against the same calls with the sets one after another on one stream."""
+    import torch
+    from cumayo_amd import Engine
+    from cumayo_amd.mixed import MixedBatch
+    weights = weights or {1: 0.4, 2: 0.2, 3: 0.2, 5: 0.2}
+    engines, prepared = {}, {}
+    for v in weights:
+        e = eng1 if v == eng1.version else Engine(v)
+        esk, epk = e.keypair_tables(keys, rand_index=0)
+        engines[v], prepared[v] = e, e.prepare_tables(esk, epk)
+    flat, offs, lens, kidx = make_workload(n, keys, 32, 1024, tag="mixed-batch")
+    cuts = np.cumsum([weights[v] for v in weights])
+    sets = np.array(list(weights))
+    u = np.array([int.from_bytes(hashlib.shake_256(b"mayo-bench/mixed-batch/set" + j.to_bytes(8, "little"))
+                                 .digest(8), "little") for j in range(n)], dtype=np.float64) / 2.0**64
+    set_of = sets[np.minimum(np.searchsorted(cuts, u, side="right"), len(sets) - 1)].astype(np.int32)
+    dev = eng1.device
+    d_msgs = torch.from_numpy(np.array(flat)).to(dev)
+    mb = MixedBatch(engines, prepared)
+    rates = {}
+    outs = {}
+    for conc in (True, False):
+        sig = mb.sign(set_of, kidx, d_msgs, offs, lens, concurrent=conc)
+        res = mb.verify(set_of, kidx, d_msgs, offs, lens, {v: s for v, (p, s) in sig.items()}, concurrent=conc)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sig = mb.sign(set_of, kidx, d_msgs, offs, lens, concurrent=conc)
+        torch.cuda.synchronize(dev)
+        ts = (time.perf_counter() - t0) / steps
+        s_in = {v: s for v, (p, s) in sig.items()}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res = mb.verify(set_of, kidx, d_msgs, offs, lens, s_in, concurrent=conc)
+        torch.cuda.synchronize(dev)
+        tv = (time.perf_counter() - t0) / steps
+        rates[conc] = (n / ts, n / tv)
+        outs[conc] = ({v: s.cpu().numpy() for v, (p, s) in sig.items()}, res.cpu().numpy())
+    same = all(np.array_equal(outs[True][0][v], outs[False][0][v]) for v in outs[True][0])
+    return {"items": n, "mix": {f"MAYO_{v}": int((set_of == v).sum()) for v in weights},
+            "messages": "uniform [32, 1024] B, key j mod 64 of the item's set",
+            "sign_per_s": round(rates[True][0], 1), "verify_per_s": round(rates[True][1], 1),
+            "sets_in_sequence": {"sign_per_s": round(rates[False][0], 1),
+                                 "verify_per_s": round(rates[False][1], 1)},
+            "identical_outputs": bool(same), "all_valid": bool((outs[True][1] == 0).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +222,8 @@ def main():
     ap.add_argument("--flow-batch", type=int, default=20000,
                     help="batch of the reference-flow program (keypair, signWithesk, "
                          "verifyWithepk over 10 streams, 32-byte messages); 0 skips it")
+    ap.add_argument("--mixed-batch", type=int, default=65536,
+                    help="items of the mixed-parameter-set batch run (MAYO_1/2/3/5 = 40/20/20/20); 0 skips it")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the compiled default launch variants (no launch autotuning)")
     ap.add_argument("--autotune-reps", type=int, default=3,
@@ -395,6 +449,10 @@ def main():
             "traffic_unit": "bytes per launch (PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
         },
     }
+
+    if args.mixed_batch > 0:
+        print(f"[bench] mixed-set batch of {args.mixed_batch}", file=sys.stderr, flush=True)
+        out["mixed_set_batch"] = mixed_batch_run(eng, args.mixed_batch, max(1, args.steps // 2))
 
     if args.flow_batch > 0:
         # the reference's own benchmark method through the drop-in C ABI

@@ -1,12 +1,15 @@
 """Mixed parameter sets on one GPU.
 
 Each MAYO parameter set is its own library (one build per set, as in the
-reference), so a mixed stream is served by one continuous-batching server per
-set, all sharing the device: a request is routed to its set's server, the
-servers' batches overlap on the GPU, and latencies are merged. Under load,
-each set gets its own open-loop Poisson stream (rate x weight); the
-superposition of independent Poisson streams is the Poisson stream of the
-whole mix.
+reference), so its kernels are compiled for that set's sizes. A batch that
+mixes sets (MixedBatch) is split by set on the host and each set's part runs
+through its library on a HIP stream of its own, so the sets' kernels overlap
+on the device. A mixed request stream is served by one continuous-batching
+server per set (MixedServer), all sharing the device: a request is routed to
+its set's server, the servers' batches overlap on the GPU, and latencies are
+merged. Under load, each set gets its own open-loop Poisson stream (rate x
+weight); the superposition of independent Poisson streams is the Poisson
+stream of the whole mix.
 """
 from __future__ import annotations
 
@@ -23,6 +26,112 @@ def percentiles(lat_ns: np.ndarray) -> dict:
     q = np.percentile(lat_ns, [50, 90, 99]) / 1e6
     return {"p50_ms": float(q[0]), "p90_ms": float(q[1]), "p99_ms": float(q[2]),
             "max_ms": float(lat_ns.max() / 1e6)}
+
+
+class MixedBatch:
+    """One batch whose items use different parameter sets.
+
+    engines[v] is the Engine of set v and prepared[v] its prepared key tables
+    (prep_esk, prep_epk), either of which may be None. Item i uses set
+    set_of[i], key key_idx[i] of that set's tables and message bytes
+    msgs[msg_off[i] : msg_off[i] + msg_len[i]] of one flat device buffer.
+    sign() and verify() split the items by set (host index arrays), gather
+    each set's key indices, offsets and lengths to the device, and launch
+    every set's hashing and linear-algebra passes on its own stream after
+    the caller's stream; the caller's stream then waits for all of them.
+    Results equal those of one single-set batch per set."""
+
+    def __init__(self, engines: dict, prepared: dict):
+        self.engines, self.prepared = engines, prepared
+        eng = next(iter(engines.values()))
+        self.torch = eng.torch
+        # one stream per set, default priority
+        self.streams = {v: self.torch.cuda.Stream(device=e.device) for v, e in engines.items()}
+
+    def split(self, set_of) -> dict:
+        """{v: positions of the items of set v, in batch order}."""
+        set_of = np.asarray(set_of)
+        return {v: np.flatnonzero(set_of == v) for v in self.engines if (set_of == v).any()}
+
+    def _parts(self, set_of, key_idx, msg_off, msg_len):
+        torch = self.torch
+        parts = {}
+        for v, pos in self.split(set_of).items():
+            dev = self.engines[v].device
+            t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev, non_blocking=False)
+            parts[v] = (pos, t(np.asarray(key_idx, np.int32)[pos]), t(np.asarray(msg_off, np.int64)[pos]),
+                        t(np.asarray(msg_len, np.int32)[pos]))
+        unknown = set(np.unique(np.asarray(set_of)).tolist()) - set(self.engines)
+        if unknown:
+            raise ValueError(f"no engine for parameter sets {sorted(unknown)}")
+        return parts
+
+    def _launch(self, parts, body, concurrent=True):
+        torch = self.torch
+        cur = torch.cuda.current_stream()
+        if not concurrent:
+            # every set on the caller's stream, one after another
+            for v, part in parts.items():
+                body(v, part, cur)
+            return
+        for v, part in parts.items():
+            st = self.streams[v]
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                body(v, part, st)
+        for v in parts:
+            cur.wait_stream(self.streams[v])
+
+    def sign(self, set_of, key_idx, msgs, msg_off, msg_len, concurrent=True) -> dict:
+        """Returns {v: (positions, sigs)}: sigs[k] (device, sig_bytes of set v)
+        is the signature of item positions[k]. concurrent=False runs the sets
+        one after another on the caller's stream (same results)."""
+        torch = self.torch
+        parts = self._parts(set_of, key_idx, msg_off, msg_len)
+        out = {}
+
+        def body(v, part, st):
+            eng = self.engines[v]
+            pos, dk, do, dl = part
+            n = pos.size
+            sigs = torch.empty((n, eng.sizes.sig_bytes), dtype=torch.uint8, device=eng.device)
+            ws = eng.sign_workspace(n)
+            eng.sign_prepared(self.prepared[v][0], dk, msgs, do, dl, sigs, ws, stream=st)
+            for x in (dk, do, dl, sigs, ws):
+                x.record_stream(st)
+            msgs.record_stream(st)
+            out[v] = (pos, sigs)
+
+        self._launch(parts, body, concurrent)
+        return out
+
+    def verify(self, set_of, key_idx, msgs, msg_off, msg_len, sigs: dict, concurrent=True):
+        """sigs[v]: (n_v, sig_bytes) device signatures of set v's items in
+        batch order (as sign() returns them). Returns a device uint8 tensor of
+        results in batch order (0 valid, 1 invalid)."""
+        torch = self.torch
+        parts = self._parts(set_of, key_idx, msg_off, msg_len)
+        dev = next(iter(self.engines.values())).device
+        results = torch.full((len(set_of),), 0xEE, dtype=torch.uint8, device=dev)
+        per = {}
+
+        def body(v, part, st):
+            eng = self.engines[v]
+            pos, dk, do, dl = part
+            n = pos.size
+            res = torch.empty(n, dtype=torch.uint8, device=eng.device)
+            ws = eng.verify_workspace(n)
+            eng.verify_prepared(self.prepared[v][1], dk, msgs, do, dl, sigs[v], res, ws, stream=st)
+            for x in (dk, do, dl, res, ws):
+                x.record_stream(st)
+            msgs.record_stream(st)
+            sigs[v].record_stream(st)
+            per[v] = (pos, res)
+
+        self._launch(parts, body, concurrent)
+        for v, (pos, res) in per.items():
+            results[torch.from_numpy(pos).to(dev)] = res
+        return results
 
 
 class MixedServer:

@@ -1,0 +1,69 @@
+"""GPU: one batch that mixes the four parameter sets (MixedBatch). Every
+item's signature equals the one its set's single-set batch gives (and the
+oracle's, sampled); verification of the mixed batch accepts them and
+rejects mutated ones, in batch order."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_set_batch_equals_single_set_batches(engine_factory):
+    import torch
+    from cumayo_amd.mixed import MixedBatch
+    sets = (1, 2, 3, 5)
+    engines, prepared, tables = {}, {}, {}
+    for v in sets:
+        eng = engine_factory(v)
+        esk, epk = eng.keypair_tables(4, rand_index=60 + v)
+        engines[v], tables[v] = eng, esk
+        prepared[v] = eng.prepare_tables(esk, epk)
+    n = 700
+    rng = np.random.default_rng(2024)
+    set_of = rng.choice(sets, size=n, p=[0.4, 0.2, 0.2, 0.2]).astype(np.int32)
+    lens = rng.integers(0, 900, n).astype(np.int32)
+    lens[:3] = [0, 136, 137]
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+    kidx = rng.integers(0, 4, n).astype(np.int32)
+    dev = engines[1].device
+    d_msgs = torch.from_numpy(flat).to(dev)
+    mb = MixedBatch(engines, prepared)
+    print(f"[mixed-batch] n={n} sets {np.bincount(set_of).tolist()}", flush=True)
+    out = mb.sign(set_of, kidx, d_msgs, offs, lens)
+    torch.cuda.synchronize()
+    assert sorted(out) == list(sets)
+    seen = np.zeros(n, bool)
+    for v, (pos, sigs) in out.items():
+        eng = engines[v]
+        assert np.all(set_of[pos] == v) and np.all(np.diff(pos) > 0)
+        seen[pos] = True
+        # the same items as a single-set batch
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        ref = torch.zeros_like(sigs)
+        eng.sign_prepared(prepared[v][0], t(kidx[pos]), d_msgs, t(offs[pos]), t(lens[pos]), ref,
+                          eng.sign_workspace(pos.size))
+        torch.cuda.synchronize()
+        assert torch.equal(sigs, ref), f"MAYO_{v}"
+        esk = tables[v].cpu().numpy()
+        g = sigs.cpu().numpy()
+        for k in (0, pos.size - 1):
+            i = pos[k]
+            m = flat[offs[i]:offs[i] + lens[i]].tobytes()
+            assert g[k].tobytes() == O.sign(v, esk[kidx[i]].tobytes(), m)[0], (v, i)
+    assert seen.all()
+    # verification of the mixed batch, with every fifth signature mutated
+    bad = {}
+    expect = np.zeros(n, np.uint8)
+    for v, (pos, sigs) in out.items():
+        b = sigs.clone()
+        b[::5, 1] ^= 0x40
+        bad[v] = b
+        expect[pos[::5]] = 1
+    res = mb.verify(set_of, kidx, d_msgs, offs, lens, {v: s for v, (p, s) in out.items()})
+    res_bad = mb.verify(set_of, kidx, d_msgs, offs, lens, bad)
+    torch.cuda.synchronize()
+    assert res.cpu().numpy().tolist() == [0] * n
+    assert np.array_equal(res_bad.cpu().numpy(), expect)
