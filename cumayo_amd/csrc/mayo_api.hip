@@ -356,13 +356,70 @@ namespace {
 // waves per SIMD it is built for; for every table form, the item order of
 // the one-wave-per-item kernels (0 batch order, 1 XCD-aware key order).
 struct LaunchCfg {
-    int prep, solve, sign_order, verify, verify_order;
+    int prep, solve, sign_order, verify, verify_order, sign_chunks;
 };
 std::atomic<int> g_prep{wv::PREP_OCCS[0]}, g_solve{wv::SOLVE_OCCS[0]}, g_sign_order{0},
-    g_verify{wv::VERIFY_OCCS[0]}, g_verify_order{0};
+    g_verify{wv::VERIFY_OCCS[0]}, g_verify_order{0}, g_sign_chunks{1};
 
 LaunchCfg current_cfg() {
-    return {g_prep.load(), g_solve.load(), g_sign_order.load(), g_verify.load(), g_verify_order.load()};
+    return {g_prep.load(),   g_solve.load(),        g_sign_order.load(),
+            g_verify.load(), g_verify_order.load(), g_sign_chunks.load()};
+}
+
+// Stage 2 of signing in two parts needs at least this many items (each
+// part then still holds several rounds of waves per SIMD).
+constexpr size_t CHUNK_MIN = 16384;
+
+// This thread's helper stream on the current device (default priority),
+// created on first use and kept for the life of the thread.
+hipStream_t aux_stream() {
+    thread_local hipStream_t streams[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+        streams[dev] = nullptr;
+    return streams[dev];
+}
+
+// Stage 2 of signing: attempt 0 in two kernels tuned for their register
+// needs (prep_k, solve_k), over the whole batch or, with parts = 2, over
+// two halves, the second on the helper stream after a fork from `stream`
+// and joined back before the retry pass; the kernels of one half then fill
+// the other half's tails. Then the rank failures (about 1 in 4,000) go
+// through the retry pass over the failure list.
+template <class KL, class PK, class SK>
+hipError_t launch_sign_stage2(PK prep_k, SK solve_k, const TableSign &acc, size_t n, int ordered, int fail_first,
+                              int parts, hipStream_t stream) {
+    hipStream_t aux = parts > 1 ? aux_stream() : nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    if (parts > 1 && aux && hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&join, hipEventDisableTiming) == hipSuccess) {
+        // the list is emptied once, ahead of both halves' solve kernels
+        hipLaunchKernelGGL(wv::k_reset_list<TableSign>, dim3(1), dim3(64), 0, stream, acc);
+        hipError_t e = hipEventRecord(fork, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+        const size_t h = n / 2;
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(prep_k, dim3((unsigned)h), dim3(64), 0, stream, acc, n, ordered, (size_t)0, 0);
+            hipLaunchKernelGGL(solve_k, dim3((unsigned)h), dim3(64), 0, stream, acc, n, fail_first, ordered,
+                               (size_t)0);
+            hipLaunchKernelGGL(prep_k, dim3((unsigned)(n - h)), dim3(64), 0, aux, acc, n, ordered, h, 0);
+            hipLaunchKernelGGL(solve_k, dim3((unsigned)(n - h)), dim3(64), 0, aux, acc, n, fail_first, ordered, h);
+            e = hipEventRecord(join, aux);
+            if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
+        }
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+        if (e != hipSuccess) return e;
+    } else {
+        if (fork) (void)hipEventDestroy(fork);
+        hipLaunchKernelGGL(prep_k, dim3((unsigned)n), dim3(64), 0, stream, acc, n, ordered, (size_t)0, 1);
+        hipLaunchKernelGGL(solve_k, dim3((unsigned)n), dim3(64), 0, stream, acc, n, fail_first, ordered, (size_t)0);
+    }
+    const unsigned retry_grid = (unsigned)(n < 256 ? n : 256);
+    hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3(retry_grid), dim3(64 * wv::RNW), 0, stream, acc, n,
+                       fail_first);
+    return hipGetLastError();
 }
 
 template <size_t N>
@@ -403,34 +460,31 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
         if (stages & 4) {
             k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
-            // attempt 0 in two kernels tuned for their register needs, then
-            // the rank failures (about 1 in 4,000) through the full loop
             const int fail_first = (stages >> 8) & 0xFF;
             const int ordered = cfg.sign_order && n > 64;
+            const int parts = cfg.sign_chunks > 1 && n >= CHUNK_MIN ? 2 : 1;
             if (ordered) k_key_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
-            const unsigned grid = (unsigned)n;
             if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
                 // prepared keys: the variants the launch autotuner chooses among
-                const bool found =
-                    with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto O) {
-                        wv::k_sign_prep<TableSign, KL, decltype(O)::value><<<grid, 64, 0, stream>>>(acc, n, ordered);
-                    }) &&
-                    with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto O) {
-                        wv::k_sign_solve<TableSign, KL::PLANAR, decltype(O)::value>
-                            <<<grid, 64, 0, stream>>>(acc, n, fail_first, ordered);
+                bool found = false;
+                with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto P) {
+                    with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto Q) {
+                        found = true;
+                        e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL, decltype(P)::value>,
+                                                   &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
+                                                   acc, n, ordered, fail_first, parts, stream);
                     });
+                });
                 if (!found) {
                     note(hipErrorInvalidValue);
                     return (int)hipErrorInvalidValue;
                 }
             } else {
-                wv::k_sign_prep<TableSign, KL><<<grid, 64, 0, stream>>>(acc, n, ordered);
-                wv::k_sign_solve<TableSign, KL::PLANAR><<<grid, 64, 0, stream>>>(acc, n, fail_first, ordered);
+                e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL>, &wv::k_sign_solve<TableSign, KL::PLANAR>,
+                                           acc, n, ordered, fail_first, parts, stream);
             }
-            const unsigned retry_grid = (unsigned)(n < 256 ? n : 256);
-            wv::k_sign_retry<TableSign, KL><<<retry_grid, 64 * wv::RNW, 0, stream>>>(acc, n, fail_first);
         }
-        e = hipGetLastError();
+        if (e == hipSuccess) e = hipGetLastError();
     }
     note(e);
     return (int)e;
@@ -629,6 +683,9 @@ namespace {
 // instantiated graph; null (and the error noted) on failure.
 template <class F>
 cumayo_graph *capture(F &&launch) {
+    // a two-part signing pass forks onto this thread's helper stream: create
+    // it before capture starts
+    (void)aux_stream();
     hipStream_t s = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -687,18 +744,20 @@ void cumayo_get_launch_config(cumayo_launch_config *out) {
     out->sign_order = c.sign_order;
     out->verify = c.verify;
     out->verify_order = c.verify_order;
+    out->sign_chunks = c.sign_chunks;
 }
 
 int cumayo_set_launch_config(const cumayo_launch_config *cfg) {
     if (!cfg || !listed(wv::PREP_OCCS, cfg->sign_prep) || !listed(wv::SOLVE_OCCS, cfg->sign_solve) ||
         !listed(wv::VERIFY_OCCS, cfg->verify) || (unsigned)cfg->sign_order > 1u ||
-        (unsigned)cfg->verify_order > 1u)
+        (unsigned)cfg->verify_order > 1u || cfg->sign_chunks < 1 || cfg->sign_chunks > 2)
         return (int)hipErrorInvalidValue;
     g_prep.store(cfg->sign_prep);
     g_solve.store(cfg->sign_solve);
     g_sign_order.store(cfg->sign_order);
     g_verify.store(cfg->verify);
     g_verify_order.store(cfg->verify_order);
+    g_sign_chunks.store(cfg->sign_chunks);
     return 0;
 }
 
@@ -706,12 +765,14 @@ int cumayo_launch_variants(int field, int32_t *out, int max) {
     const int *v = nullptr;
     int cnt = 0;
     static const int orders[2] = {0, 1};
+    static const int chunks[2] = {1, 2};
     auto use = [&](const auto &arr) { v = arr, cnt = (int)(sizeof(arr) / sizeof(arr[0])); };
     switch (field) {
         case CUMAYO_FIELD_SIGN_PREP: use(wv::PREP_OCCS); break;
         case CUMAYO_FIELD_SIGN_SOLVE: use(wv::SOLVE_OCCS); break;
         case CUMAYO_FIELD_VERIFY: use(wv::VERIFY_OCCS); break;
         case CUMAYO_FIELD_ORDER: use(orders); break;
+        case CUMAYO_FIELD_SIGN_CHUNKS: use(chunks); break;
         default: return -1;
     }
     for (int i = 0; i < cnt && i < max && out; ++i) out[i] = v[i];
@@ -743,20 +804,24 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
         err = sign_table_impl<wv::PrepKeys>(acc, n, stream, 1, cfg);  // hashing pass, once
         double best = 1e30;
         LaunchCfg bc = cfg;
-        for (int order = 0; order < 2; ++order)
-            for (int p : wv::PREP_OCCS)
-                for (int so : wv::SOLVE_OCCS) {
-                    if (err) break;
-                    LaunchCfg c = cfg;
-                    c.prep = p, c.solve = so, c.sign_order = order;
-                    const double t = time_min_ms(
-                        stream, a, b, reps, [&] { return sign_table_impl<wv::PrepKeys>(acc, n, stream, 2, c); },
-                        &err);
-                    ++timed;
-                    if (p == wv::PREP_OCCS[0] && so == wv::SOLVE_OCCS[0] && order == 0) out->sign_default_ms = t;
-                    if (t < best) best = t, bc = c;
-                }
-        cfg.prep = bc.prep, cfg.solve = bc.solve, cfg.sign_order = bc.sign_order;
+        // the split into two parts only applies from CHUNK_MIN items on
+        const int max_chunks = n >= CHUNK_MIN ? 2 : 1;
+        for (int chunks = 1; chunks <= max_chunks; ++chunks)
+            for (int order = 0; order < 2; ++order)
+                for (int p : wv::PREP_OCCS)
+                    for (int so : wv::SOLVE_OCCS) {
+                        if (err) break;
+                        LaunchCfg c = cfg;
+                        c.prep = p, c.solve = so, c.sign_order = order, c.sign_chunks = chunks;
+                        const double t = time_min_ms(
+                            stream, a, b, reps,
+                            [&] { return sign_table_impl<wv::PrepKeys>(acc, n, stream, 2, c); }, &err);
+                        ++timed;
+                        if (p == wv::PREP_OCCS[0] && so == wv::SOLVE_OCCS[0] && order == 0 && chunks == 1)
+                            out->sign_default_ms = t;
+                        if (t < best) best = t, bc = c;
+                    }
+        cfg.prep = bc.prep, cfg.solve = bc.solve, cfg.sign_order = bc.sign_order, cfg.sign_chunks = bc.sign_chunks;
         out->sign_ms = best;
     }
     if (do_verify && !err) {
@@ -788,6 +853,7 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
         g_sign_order.store(cfg.sign_order);
         g_verify.store(cfg.verify);
         g_verify_order.store(cfg.verify_order);
+        g_sign_chunks.store(cfg.sign_chunks);
     }
     cumayo_get_launch_config(&out->best);
     out->configs_timed = timed;

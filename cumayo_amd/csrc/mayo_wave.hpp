@@ -61,11 +61,13 @@ inline constexpr int SOLVE_OCCS[3] = {SOLVE_OCC, M <= 80 ? 6 : 2, 4};
 inline constexpr int VERIFY_OCCS[3] = {qf_code(QF_RT, QF_OCC), qf_code(QF_RT, QF_OCC - 1),
                                        qf_code(1, K >= 12 ? 3 : QF_OCC + 1)};
 
-// Item of this workgroup: blockIdx.x, or its entry in the XCD-aware order
-// (k_key_order) when `ordered` is set.
+// Item of this workgroup: position base + blockIdx.x of the batch, or the
+// item at that position of the XCD-aware order (k_key_order) when `ordered`
+// is set. A launch over part of a batch passes the part's first position.
 template <class Acc>
-__device__ __forceinline__ size_t item_of(const Acc &acc, int ordered) {
-    return ordered ? acc.order(blockIdx.x) : blockIdx.x;
+__device__ __forceinline__ size_t item_of(const Acc &acc, int ordered, size_t base = 0) {
+    const size_t k = base + blockIdx.x;
+    return ordered ? acc.order(k) : k;
 }
 // column c lives in lane col_lane(c), set col_set(c); the first SHIFT columns
 // go to the last set so that only one set stays active for most pivots
@@ -1015,15 +1017,24 @@ __global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, i
     }
 }
 
+// Empties the failure list of the solve kernels before the parts of a batch
+// whose stage 2 runs as several launches.
+template <class Acc>
+__global__ void k_reset_list(Acc acc) {
+    if (threadIdx.x == 0) *(uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT) = 0;
+}
+
 // Batch form, attempt 0, part 1: M, U and y, handed over through the
 // workspace. Register-heavy (the pair accumulators of the quadratic form).
 template <class Acc, class KL = RefKeys, int CODE = qf_code(QF_RT, QF_OCC)>
-__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0) {
+__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0, size_t base = 0,
+                                                             int reset_list = 1) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
-    SignItem it = acc(item_of(acc, ordered));
-    // the failure list of the solve kernel starts empty
-    if (blockIdx.x == 0 && lane == 0) *(uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT) = 0;
+    SignItem it = acc(item_of(acc, ordered, base));
+    // the failure list of the solve kernel starts empty (a launch over part
+    // of a batch leaves that to k_reset_list, ahead of every part)
+    if (reset_list && blockIdx.x == 0 && lane == 0) *(uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT) = 0;
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     __syncthreads();
@@ -1043,10 +1054,11 @@ __global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, 
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
 // PL: the M vectors in the workspace are planar (prepared keys).
 template <class Acc, bool PL, int OCC = SOLVE_OCC>
-__global__ void __launch_bounds__(64, OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0, int ordered = 0) {
+__global__ void __launch_bounds__(64, OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0, int ordered = 0,
+                                                       size_t base = 0) {
     __shared__ SolveShared S;
     const int lane = threadIdx.x;
-    const size_t item = item_of(acc, ordered);
+    const size_t item = item_of(acc, ordered, base);
     SignItem it = acc(item);
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     planes_of_M<PL>((const uint32_t *)(it.scratch + SCR_MV), S.Mb);

@@ -70,7 +70,7 @@ class Sizes:
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("sign_prep", ctypes.c_int32), ("sign_solve", ctypes.c_int32), ("sign_order", ctypes.c_int32),
-                ("verify", ctypes.c_int32), ("verify_order", ctypes.c_int32)]
+                ("verify", ctypes.c_int32), ("verify_order", ctypes.c_int32), ("sign_chunks", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {f[0]: int(getattr(self, f[0])) for f in self._fields_}
@@ -82,7 +82,7 @@ class TuneReport(ctypes.Structure):
                 ("configs_timed", ctypes.c_int32)]
 
 
-LAUNCH_FIELDS = {"sign_prep": 0, "sign_solve": 1, "verify": 2, "order": 3}
+LAUNCH_FIELDS = {"sign_prep": 0, "sign_solve": 1, "verify": 2, "order": 3, "sign_chunks": 4}
 
 # (name, argtypes) of every symbol the header declares
 _VP = ctypes.c_void_p

@@ -306,12 +306,18 @@ typedef struct {
     int32_t sign_order;   /* 0 batch order, 1 XCD-aware key order */
     int32_t verify;       /* verification kernel (10 x rows per lane + waves per SIMD) */
     int32_t verify_order;
+    int32_t sign_chunks;  /* 1, or 2: from 16,384 items on, the signing linear-algebra pass
+                             runs as two halves on two streams (the second on a helper
+                             stream of the calling thread, forked from and joined back to
+                             the caller's stream), so that each half fills the other's
+                             kernel tails */
 } cumayo_launch_config;
 
 #define CUMAYO_FIELD_SIGN_PREP 0
 #define CUMAYO_FIELD_SIGN_SOLVE 1
 #define CUMAYO_FIELD_VERIFY 2
 #define CUMAYO_FIELD_ORDER 3
+#define CUMAYO_FIELD_SIGN_CHUNKS 4
 
 void cumayo_get_launch_config(cumayo_launch_config *out);
 /* Returns hipErrorInvalidValue (nothing changed) for a value that is not a

@@ -79,20 +79,23 @@ def test_launch_variants_and_config(version):
         variants[name] = [buf[i] for i in range(cnt)]
         assert len(set(variants[name])) == cnt, (name, variants[name])
     assert lib.cumayo_launch_variants(99, None, 0) == -1
-    assert variants["order"] == [0, 1]
+    assert variants["order"] == [0, 1] and variants["sign_chunks"] == [1, 2]
     cfg = LaunchConfig()
     lib.cumayo_get_launch_config(ctypes.byref(cfg))
     assert cfg.as_dict() == {"sign_prep": variants["sign_prep"][0], "sign_solve": variants["sign_solve"][0],
-                             "sign_order": 0, "verify": variants["verify"][0], "verify_order": 0}
+                             "sign_order": 0, "verify": variants["verify"][0], "verify_order": 0,
+                             "sign_chunks": 1}
     # quadratic-form variants are coded 10 x rows per lane + waves per SIMD
     for code in variants["sign_prep"] + variants["verify"]:
         assert code // 10 in (1, 2) and 1 <= code % 10 <= 8, code
-    alt = LaunchConfig(variants["sign_prep"][1], variants["sign_solve"][2], 1, variants["verify"][1], 1)
+    alt = LaunchConfig(variants["sign_prep"][1], variants["sign_solve"][2], 1, variants["verify"][1], 1, 2)
     assert lib.cumayo_set_launch_config(ctypes.byref(alt)) == 0
     got = LaunchConfig()
     lib.cumayo_get_launch_config(ctypes.byref(got))
     assert got.as_dict() == alt.as_dict()
-    bad = LaunchConfig(variants["sign_prep"][0] + 50, variants["sign_solve"][0], 0, variants["verify"][0], 0)
+    bad = LaunchConfig(variants["sign_prep"][0] + 50, variants["sign_solve"][0], 0, variants["verify"][0], 0, 1)
+    assert lib.cumayo_set_launch_config(ctypes.byref(bad)) != 0
+    bad = LaunchConfig(variants["sign_prep"][0], variants["sign_solve"][0], 0, variants["verify"][0], 0, 3)
     assert lib.cumayo_set_launch_config(ctypes.byref(bad)) != 0
     lib.cumayo_get_launch_config(ctypes.byref(got))
     assert got.as_dict() == alt.as_dict()

@@ -106,7 +106,9 @@ def test_autotune_keeps_a_compiled_variant_and_the_results(engine_factory, resto
     assert best == eng.launch_config()
     assert best["sign_prep"] in var["sign_prep"] and best["sign_solve"] in var["sign_solve"]
     assert best["verify"] in var["verify"] and best["sign_order"] in (0, 1) and best["verify_order"] in (0, 1)
-    assert rep["configs_timed"] == 2 * len(var["sign_prep"]) * len(var["sign_solve"]) + 2 * len(var["verify"])
+    chunk_options = 1 if n < 16384 else 2  # the two-part launch applies from 16,384 items on
+    assert rep["configs_timed"] == (2 * chunk_options * len(var["sign_prep"]) * len(var["sign_solve"]) +
+                                    2 * len(var["verify"]))
     assert 0 < rep["sign_ms"] <= rep["sign_default_ms"] and 0 < rep["verify_ms"] <= rep["verify_default_ms"]
     assert torch.equal(sigs, ref)
     assert res.cpu().numpy().tolist() == [0] * n
@@ -167,6 +169,58 @@ def test_graph_replays_match_eager(engine_factory, forced):
     esk = b["esk"].cpu().numpy()
     r = ref.cpu().numpy()
     for i in (0, n // 2, n - 1):
+        m = b["flat"][b["offs"][i]:b["offs"][i] + b["lens"][i]].tobytes()
+        osig, oatt = O.sign(v, esk[b["kidx"][i]].tobytes(), m, fail_first=forced)
+        assert r[i].tobytes() == osig and int(att[i]) == oatt, i
+
+
+@pytest.mark.parametrize("forced", (0, 1), ids=("plain", "forced-retries"))
+def test_two_part_stage2_matches_one_part(engine_factory, restore_config, forced):
+    """From 16,384 items on, sign_chunks = 2 runs the linear-algebra pass as
+    two halves on two streams with one failure list: the signatures and
+    attempt counts equal the one-part launch's, also when every item goes
+    through the retry pass, and also replayed from a graph captured across
+    both streams."""
+    import torch
+    v = 1
+    eng = restore_config(engine_factory(v))
+    z = eng.sizes
+    n = 20000
+    b = _batch(eng, n, 8, 90 + forced, maxlen=300)
+    stages = 3 | (forced << 8)
+    eng.set_launch_config(sign_chunks=1)
+    ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=eng.device)
+    ws1 = eng.sign_workspace(n)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], ref, ws1, stages=stages)
+    torch.cuda.synchronize()
+    att = eng.sign_attempts(ws1, n)
+    assert (att >= forced + 1).all()
+    for order in (0, 1):
+        eng.set_launch_config(sign_chunks=2, sign_order=order)
+        print(f"[tuning] two-part stage 2 n={n} forced={forced} order={order}", flush=True)
+        out = torch.full_like(ref, 0x5A)
+        ws2 = eng.sign_workspace(n)
+        eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], out, ws2, stages=stages)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), order
+        assert np.array_equal(eng.sign_attempts(ws2, n), att), order
+    eng.set_launch_config(sign_chunks=2, sign_order=0)
+    gsig = torch.zeros_like(ref)
+    gws = eng.sign_workspace(n)
+    g = eng.sign_graph(b["pesk"], b["k"], b["m"], b["o"], b["l"], gsig, gws, stages=stages)
+    try:
+        for rep in range(2):
+            gsig.zero_()
+            torch.cuda.synchronize()
+            g.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(gsig, ref), f"graph replay {rep}"
+            assert np.array_equal(eng.sign_attempts(gws, n), att)
+    finally:
+        g.close()
+    esk = b["esk"].cpu().numpy()
+    r = ref.cpu().numpy()
+    for i in (0, n // 2 - 1, n // 2, n - 1):
         m = b["flat"][b["offs"][i]:b["offs"][i] + b["lens"][i]].tobytes()
         osig, oatt = O.sign(v, esk[b["kidx"][i]].tobytes(), m, fail_first=forced)
         assert r[i].tobytes() == osig and int(att[i]) == oatt, i
