@@ -450,6 +450,21 @@ def main():
         },
     }
 
+    # the VALU peak measured on this GPU: a stream of independent v_bitop3_b32
+    # AND-XORs, the multiply-accumulate step of the kernels (2 GF(16) MACs per
+    # lane-op), next to the nominal 1 op per 4 cycles per SIMD at 2.4 GHz
+    import subprocess
+    from cumayo_amd import build as _build
+    if os.path.exists(_build.peak_path()):
+        pk = subprocess.run([_build.peak_path()], capture_output=True, text=True, timeout=60)
+        if pk.returncode == 0:
+            meas = json.loads(pk.stdout.strip().splitlines()[-1])
+            out["roofline"]["peak_measured"] = {
+                "program": "cumayo_amd/lib/valu_peak (v_bitop3_b32 stream, 8 waves per SIMD)",
+                "gf16_tmacs": meas["gf16_tmacs"], "cycles_per_wave_op": meas["cycles_per_wave_op"],
+                "clock_ghz": meas["clock_ghz"],
+                "frac_of_measured": round(achieved_tmacs / meas["gf16_tmacs"], 4)}
+
     if args.mixed_batch > 0:
         print(f"[bench] mixed-set batch of {args.mixed_batch}", file=sys.stderr, flush=True)
         out["mixed_set_batch"] = mixed_batch_run(eng, args.mixed_batch, max(1, args.steps // 2))
@@ -459,8 +474,6 @@ def main():
         # (a C++ program linked against the library): per-item key expansion
         # and both PCIe copies inside the timed region. Run before the serve
         # runs, on an otherwise idle device.
-        import subprocess
-        from cumayo_amd import build as _build
         print(f"[bench] reference flow, batch {args.flow_batch}", file=sys.stderr, flush=True)
         res = subprocess.run([_build.flow_path(v), str(args.flow_batch), "10", "10", "32"],
                              capture_output=True, text=True, timeout=300)

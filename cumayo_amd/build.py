@@ -60,12 +60,33 @@ def build_flow(version: int) -> str:
     return out
 
 
+PEAK_SRC = os.path.join(HERE, "csrc", "valu_peak.hip")
+
+
+def peak_path() -> str:
+    return os.path.join(LIBDIR, "valu_peak")
+
+
+def build_peak() -> str:
+    """The VALU issue-rate program behind the roofline's measured peak."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = peak_path()
+    flags = [f for f in FLAGS if f not in ("-fPIC", "-shared")]
+    res = subprocess.run([HIPCC, *flags, PEAK_SRC, "-o", out + ".tmp"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for valu_peak:\n{res.stderr[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(extra=()) -> list:
     def one(v):
         lib = build_one(v, extra)
         return [lib, build_flow(v)]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        return [p for pair in ex.map(one, VERSIONS) for p in pair]
+        fut = ex.submit(build_peak)
+        built = [p for pair in ex.map(one, VERSIONS) for p in pair]
+        return built + [fut.result()]
 
 
 if __name__ == "__main__":
