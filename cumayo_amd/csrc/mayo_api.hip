@@ -352,9 +352,11 @@ size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_ST
 namespace {
 
 // Launch configuration of the batch form (process-wide for this library).
-// Over prepared keys, the compiled variant of each wave kernel, named by the
-// waves per SIMD it is built for; for every table form, the item order of
-// the one-wave-per-item kernels (0 batch order, 1 XCD-aware key order).
+// Over prepared keys, the compiled variant of each wave kernel (quadratic
+// form: 10 x rows per lane + waves per SIMD; solve: waves per SIMD) and the
+// split of signing's linear-algebra pass into one or two parts; for every
+// table form, the item order of the one-wave-per-item kernels (0 batch
+// order, 1 XCD-aware key order).
 struct LaunchCfg {
     int prep, solve, sign_order, verify, verify_order, sign_chunks;
 };
@@ -371,7 +373,9 @@ LaunchCfg current_cfg() {
 constexpr size_t CHUNK_MIN = 16384;
 
 // This thread's helper stream on the current device (default priority),
-// created on first use and kept for the life of the thread.
+// created on first use and kept for the life of the thread. Two-part passes
+// issued by one thread on different streams share it, so their second
+// halves run in the order they were issued.
 hipStream_t aux_stream() {
     thread_local hipStream_t streams[64] = {};
     int dev = 0;
