@@ -50,13 +50,16 @@ constexpr int QF_OCC = MAYO_QF_OCC ? MAYO_QF_OCC : (K <= 4 ? 4 : 3);
 constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
 // Launch variants of the batch kernels over prepared keys that the launch
 // autotuner chooses among (entry 0 is the default above). Quadratic-form
-// kernels: code 10 x rows per lane + waves per SIMD (23 = two rows per lane
-// per chunk, built for three waves per SIMD); one row halves the bit-plane
-// accumulators (more waves) but shares each column's scalar masks over half
-// as many rows. Solve kernel: waves per SIMD. Every variant computes the
-// same bytes.
+// kernels: code 10 x rows per lane + waves per SIMD they are built for (23 =
+// two rows per lane per chunk, three waves per SIMD). One row halves the
+// bit-plane accumulators (more waves) but shares each column's scalar masks
+// over half as many rows; three rows share them over more rows at fewer
+// waves and longer masked diagonals; two rows at one wave less avoid the
+// spills the default has for MAYO_3. Solve kernel: waves per SIMD. Every
+// variant computes the same bytes.
 constexpr int qf_code(int rt, int occ) { return 10 * rt + occ; }
-inline constexpr int PREP_OCCS[2] = {qf_code(QF_RT, QF_OCC), qf_code(1, K >= 12 ? 3 : QF_OCC + 1)};
+inline constexpr int PREP_OCCS[4] = {qf_code(QF_RT, QF_OCC), qf_code(1, K >= 12 ? 3 : QF_OCC + 1), qf_code(3, 2),
+                                     qf_code(QF_RT, QF_OCC - 1)};
 inline constexpr int SOLVE_OCCS[3] = {SOLVE_OCC, M <= 80 ? 6 : 2, 4};
 inline constexpr int VERIFY_OCCS[3] = {qf_code(QF_RT, QF_OCC), qf_code(QF_RT, QF_OCC - 1),
                                        qf_code(1, K >= 12 ? 3 : QF_OCC + 1)};

@@ -87,7 +87,7 @@ def test_launch_variants_and_config(version):
                              "sign_chunks": 1}
     # quadratic-form variants are coded 10 x rows per lane + waves per SIMD
     for code in variants["sign_prep"] + variants["verify"]:
-        assert code // 10 in (1, 2) and 1 <= code % 10 <= 8, code
+        assert code // 10 in (1, 2, 3) and 1 <= code % 10 <= 8, code
     alt = LaunchConfig(variants["sign_prep"][1], variants["sign_solve"][2], 1, variants["verify"][1], 1, 2)
     assert lib.cumayo_set_launch_config(ctypes.byref(alt)) == 0
     got = LaunchConfig()
