@@ -61,8 +61,8 @@ constexpr int qf_code(int rt, int occ) { return 10 * rt + occ; }
 inline constexpr int PREP_OCCS[4] = {qf_code(QF_RT, QF_OCC), qf_code(1, K >= 12 ? 3 : QF_OCC + 1), qf_code(3, 2),
                                      qf_code(QF_RT, QF_OCC - 1)};
 inline constexpr int SOLVE_OCCS[3] = {SOLVE_OCC, M <= 80 ? 6 : 2, 4};
-inline constexpr int VERIFY_OCCS[3] = {qf_code(QF_RT, QF_OCC), qf_code(QF_RT, QF_OCC - 1),
-                                       qf_code(1, K >= 12 ? 3 : QF_OCC + 1)};
+inline constexpr int VERIFY_OCCS[4] = {qf_code(QF_RT, QF_OCC), qf_code(QF_RT, QF_OCC - 1),
+                                       qf_code(1, K >= 12 ? 3 : QF_OCC + 1), qf_code(3, 2)};
 
 // Item of this workgroup: position base + blockIdx.x of the batch, or the
 // item at that position of the XCD-aware order (k_key_order) when `ordered`
