@@ -787,23 +787,23 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
         for (int pl = 0; pl < nl; ++pl) {
             const int pc = col_of(ps, pl);
             if (pc >= KO) break;
-            // every lane finds the pivot row its own column of set ps would
-            // have (a vector search: the scalar unit is the busier one);
-            // lane pl's is the one
-            int cand = -1;
+            // the pivot column f to SGPRs, then its pivot row: the lowest
+            // unused row with a nonzero entry (a scalar search, which
+            // measured faster here than every lane searching its own column
+            // on the vector unit)
+            uint32_t f[4][NG];
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
+            int pr = -1;
 #pragma unroll
             for (int g = NG - 1; g >= 0; --g) {
-                const uint32_t nz =
-                    (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~used[g];
-                cand = nz ? 32 * g + (int)__builtin_ctz(nz) : cand;
+                const uint32_t nz = (f[0][g] | f[1][g] | f[2][g] | f[3][g]) & ~used[g];
+                pr = nz ? 32 * g + (int)__builtin_ctz(nz) : pr;
             }
-            const int pr = __builtin_amdgcn_readlane(cand, pl);
+            pr = __builtin_amdgcn_readfirstlane(pr);
             if (pr >= 0) {
-                uint32_t f[4][NG];
-#pragma unroll
-                for (int d = 0; d < 4; ++d)
-#pragma unroll
-                    for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
                 const int pb = pr & 31;
                 // specialised on the pivot row's word pg (uniform), so that
                 // no per-lane word selection is needed
