@@ -28,6 +28,13 @@ def percentiles(lat_ns: np.ndarray) -> dict:
             "max_ms": float(lat_ns.max() / 1e6)}
 
 
+def split_by_set(set_of, sets) -> dict:
+    """{v: positions of the items of set v, in batch order} for the sets in
+    `sets` that occur in set_of."""
+    set_of = np.asarray(set_of)
+    return {v: np.flatnonzero(set_of == v) for v in sets if (set_of == v).any()}
+
+
 class MixedBatch:
     """One batch whose items use different parameter sets.
 
@@ -50,8 +57,7 @@ class MixedBatch:
 
     def split(self, set_of) -> dict:
         """{v: positions of the items of set v, in batch order}."""
-        set_of = np.asarray(set_of)
-        return {v: np.flatnonzero(set_of == v) for v in self.engines if (set_of == v).any()}
+        return split_by_set(set_of, self.engines)
 
     def _parts(self, set_of, key_idx, msg_off, msg_len):
         torch = self.torch
