@@ -21,6 +21,14 @@ void note(hipError_t e) {
 }
 void note_launch() { note(hipGetLastError()); }
 
+// Batch sizes above CUMAYO_MAX_BATCH are refused before anything is enqueued:
+// a grid of n workgroups of 256 work-items must stay below 2^32 work-items.
+bool batch_ok(size_t n) {
+    if (n <= CUMAYO_MAX_BATCH) return true;
+    note(hipErrorInvalidValue);
+    return false;
+}
+
 constexpr size_t A128(size_t x) { return align128(x); }
 
 // AES tables, computed from the field definition (FIPS 197 section 5.1.1):
@@ -130,7 +138,7 @@ constexpr size_t EPK_PAD = A128(EPK_LEN);
 constexpr size_t SIG_PAD = A128(SIG_BYTES);
 
 void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) {
-    if (!n) return;
+    if (!n || !batch_ok(n)) return;
     ensure_tables();
     const size_t p1_off = SK + O_BYTES;
     k_seed<<<blocks64(n), 64, 0, stream>>>(pool, pitch, n, 0, 0, 0, p1_off, SK);
@@ -143,7 +151,7 @@ void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) 
 
 void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, size_t sig_off,
                          size_t scr_off, size_t mlen, hipStream_t stream) {
-    if (!n) return;
+    if (!n || !batch_ok(n)) return;
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
@@ -152,7 +160,7 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
 }
 
 void launch_verify_pitched(uint8_t *pool, size_t pitch, size_t n, size_t mlen, hipStream_t stream) {
-    if (!n) return;
+    if (!n || !batch_ok(n)) return;
     const size_t msg_off = EPK_PAD, sig_off = EPK_PAD + A128(mlen), res_off = sig_off + SIG_PAD;
     PitchedVerify acc{pool, pitch, msg_off, sig_off, res_off, (uint32_t)mlen};
     wv::k_verify_wave<PitchedVerify><<<(unsigned)n, 64, 0, stream>>>(acc, n);
@@ -201,7 +209,7 @@ void aes128_tables_init(void) { ensure_tables(); }
 
 void crypto_sign_keypair(uint8_t *pool, size_t pitch, size_t batch_size, hipStream_t stream,
                          size_t rand_index) {
-    if (!batch_size) return;
+    if (!batch_size || !batch_ok(batch_size)) return;
     ensure_tables();
     const size_t pk_off = A128(SK), p_off = pk_off + A128(PK_BYTES);
     // seed_pk lands in pk; O is parked in the P3 area until P3 replaces it
@@ -241,7 +249,7 @@ void crypto_sign_expandpk_HtD(uint8_t *cpk, uint8_t *pool, size_t pitch, size_t 
 }
 
 void crypto_sign_expandpk(uint8_t *pool, size_t pitch, size_t batch_size, hipStream_t stream) {
-    if (!batch_size) return;
+    if (!batch_size || !batch_ok(batch_size)) return;
     ensure_tables();
     k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, P_BYTES - 16, 0);
     note_launch();
@@ -320,7 +328,7 @@ void crypto_sign_verifyWithepk_HtD(uint8_t *cpk, uint8_t *Mess, uint8_t *sig, si
 
 void crypto_sign_verifyWithepk(uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
                                hipStream_t stream) {
-    if (!batch_size) return;
+    if (!batch_size || !batch_ok(batch_size)) return;
     ensure_tables();
     k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, P_BYTES - 16, 0);
     note_launch();
@@ -552,6 +560,7 @@ int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const 
                              uint8_t *sigs, size_t n, uint8_t *workspace, hipStream_t stream,
                              int stages) {
     if (!n) return 0;
+    if (!batch_ok(n)) return (int)hipErrorInvalidValue;
     if (!esk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !workspace)
         return (int)hipErrorInvalidValue;
     TableSign acc{esk_table, esk_stride, key_idx, msgs, msg_off, msg_len, sigs, workspace};
@@ -581,6 +590,7 @@ int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const 
                          const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
                          uint8_t *workspace, hipStream_t stream, int stages) {
     if (!n) return 0;
+    if (!batch_ok(n)) return (int)hipErrorInvalidValue;
     if (!prepared || ((uintptr_t)prepared & 15) || !key_idx || !msgs || !msg_off || !msg_len || !sigs ||
         !workspace || (stages & 4))
         return (int)hipErrorInvalidValue;
@@ -593,6 +603,7 @@ int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, cons
                            uint8_t *results, size_t n, uint8_t *workspace, hipStream_t stream,
                            int stages) {
     if (!n) return 0;
+    if (!batch_ok(n)) return (int)hipErrorInvalidValue;
     if (!prepared || ((uintptr_t)prepared & 15) || !key_idx || !msgs || !msg_off || !msg_len || !sigs ||
         !results || !workspace || (stages & 4))
         return (int)hipErrorInvalidValue;
@@ -615,6 +626,7 @@ int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, cons
                                const uint8_t *sigs, uint8_t *results, size_t n, uint8_t *workspace,
                                hipStream_t stream, int stages) {
     if (!n) return 0;
+    if (!batch_ok(n)) return (int)hipErrorInvalidValue;
     if (!epk_table || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !results || !workspace)
         return (int)hipErrorInvalidValue;
     TableVerify acc{epk_table, epk_stride, key_idx, msgs, msg_off, msg_len, sigs, results, workspace};
@@ -637,6 +649,7 @@ int cumayo_crypto_sign(const uint8_t *prepared, const int32_t *key_idx, const ui
                        const int64_t *msg_off, const int32_t *msg_len, uint8_t *sm, const int64_t *sm_off,
                        size_t n, uint8_t *workspace, hipStream_t stream) {
     if (!n) return 0;
+    if (!batch_ok(n)) return (int)hipErrorInvalidValue;
     if (!sm || !sm_off) return (int)hipErrorInvalidValue;
     uint8_t *sigs = workspace + cumayo_sign_workspace_size(n);
     int e = cumayo_sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, workspace, stream, 3);
@@ -656,7 +669,9 @@ int cumayo_crypto_sign_open(const uint8_t *prepared, const int32_t *key_idx, con
                             const int64_t *out_off, int32_t *mlen_out, uint8_t *results, size_t n,
                             uint8_t *workspace, hipStream_t stream) {
     if (!n) return 0;
-    if (!sm || !sm_off || !sm_len || !msgs_out || !out_off || !results || !workspace)
+    if (!batch_ok(n)) return (int)hipErrorInvalidValue;
+    if (!prepared || ((uintptr_t)prepared & 15) || !key_idx || !sm || !sm_off || !sm_len || !msgs_out ||
+        !out_off || !results || !workspace)
         return (int)hipErrorInvalidValue;
     int32_t *len = (int32_t *)(workspace + cumayo_verify_workspace_size(n));
     uint8_t *sigs = (uint8_t *)(len + n);
@@ -788,7 +803,8 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
                     uint8_t *results, size_t n, uint8_t *sign_ws, uint8_t *verify_ws, hipStream_t stream,
                     int reps, cumayo_tune_report *out) {
     const bool do_sign = prep_esk && sign_ws, do_verify = prep_epk && verify_ws && results;
-    if (!n || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !out || (!do_sign && !do_verify) ||
+    if (!n || n > CUMAYO_MAX_BATCH || !key_idx || !msgs || !msg_off || !msg_len || !sigs || !out ||
+        (!do_sign && !do_verify) ||
         ((uintptr_t)prep_esk & 15) || ((uintptr_t)prep_epk & 15))
         return (int)hipErrorInvalidValue;
     if (reps < 1) reps = 1;

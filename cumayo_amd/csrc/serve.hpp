@@ -158,19 +158,27 @@ class Server {
     explicit Server(const cumayo_serve_config &cfg)
         : cfg_(cfg), pool_(cfg.host_threads > 0 ? cfg.host_threads - 1 : 3) {
         if (cfg_.max_batch <= 0) cfg_.max_batch = 4096;
+        if ((uint32_t)cfg_.max_batch > CUMAYO_MAX_BATCH) cfg_.max_batch = (int32_t)CUMAYO_MAX_BATCH;
         if (cfg_.n_slots <= 0) cfg_.n_slots = 3;
         if (!cfg_.slot_msg_bytes) cfg_.slot_msg_bytes = (size_t)cfg_.max_batch * 1024;
         ok(hipSetDevice(cfg_.device));
-        // resident keys in the kernel layout, built once on the device
-        if (cfg_.esk_table && cfg_.n_keys > 0) {
-            dev_alloc(&pesk_, (size_t)cfg_.n_keys * cumayo_prepared_esk_bytes());
-            if (pesk_) ok((hipError_t)cumayo_prepare_esk(cfg_.esk_table, cfg_.esk_stride, (size_t)cfg_.n_keys,
-                                                        pesk_, nullptr));
+        // resident keys in the kernel layout, built once on the device (at
+        // most PREP_KEYS keys per call of the preparation entry points)
+        constexpr size_t PREP_KEYS = 65535;
+        const size_t nk = cfg_.n_keys > 0 ? (size_t)cfg_.n_keys : 0;
+        if (cfg_.esk_table && nk) {
+            const size_t rb = cumayo_prepared_esk_bytes();
+            dev_alloc(&pesk_, nk * rb);
+            for (size_t k0 = 0; pesk_ && k0 < nk; k0 += PREP_KEYS)
+                ok((hipError_t)cumayo_prepare_esk(cfg_.esk_table + k0 * cfg_.esk_stride, cfg_.esk_stride,
+                                                  std::min(PREP_KEYS, nk - k0), pesk_ + k0 * rb, nullptr));
         }
-        if (cfg_.epk_table && cfg_.n_keys > 0) {
-            dev_alloc(&pepk_, (size_t)cfg_.n_keys * cumayo_prepared_epk_bytes());
-            if (pepk_) ok((hipError_t)cumayo_prepare_epk(cfg_.epk_table, cfg_.epk_stride, (size_t)cfg_.n_keys,
-                                                        pepk_, nullptr));
+        if (cfg_.epk_table && nk) {
+            const size_t rb = cumayo_prepared_epk_bytes();
+            dev_alloc(&pepk_, nk * rb);
+            for (size_t k0 = 0; pepk_ && k0 < nk; k0 += PREP_KEYS)
+                ok((hipError_t)cumayo_prepare_epk(cfg_.epk_table + k0 * cfg_.epk_stride, cfg_.epk_stride,
+                                                  std::min(PREP_KEYS, nk - k0), pepk_ + k0 * rb, nullptr));
         }
         ok(hipDeviceSynchronize());
         const size_t mb = (size_t)cfg_.max_batch;

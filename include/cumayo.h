@@ -32,6 +32,14 @@ extern "C" {
 #define CUMAYO_DEFAULT(x)
 #endif
 
+/* Most items (keys, signatures or verifications) one call of a batched entry
+ * point takes: HIP launches at most 2^32 work-items per grid dimension, and
+ * the workgroup-per-item kernels use 256 work-items per item. A larger batch
+ * size is refused before any work is enqueued, with hipErrorInvalidValue
+ * (returned, or recorded for cumayo_last_error by the void calls of Part 1);
+ * split larger batches into several calls. */
+#define CUMAYO_MAX_BATCH 16777215u
+
 /* ---------------- Part 1: drop-in for api.cuh ---------------- */
 
 /* api.cuh:9 -- generates batch_size key pairs into pitched pool rows:
@@ -119,7 +127,11 @@ int cumayo_last_error(void);
  * format, rows esk_stride bytes apart): item i signs msg_len[i] bytes at
  * msgs + msg_off[i] with key key_idx[i] into sigs + i*sig_bytes. All
  * pointers are device pointers; workspace holds
- * cumayo_sign_workspace_size(n) bytes. Returns 0 or a hipError_t. */
+ * cumayo_sign_workspace_size(n) bytes. Returns 0 or a hipError_t. The
+ * per-item arrays are device data and are not checked: key_idx[i] must name
+ * a row of the table, and msg_off[i] >= 0, msg_len[i] >= 0 must lie within
+ * msgs (as for every table and prepared form below); other values make the
+ * kernels read outside the caller's buffers. */
 size_t cumayo_sign_workspace_size(size_t n);
 int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t *key_idx,
                       const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
@@ -219,7 +231,7 @@ typedef struct {
     const uint8_t *esk_table; size_t esk_stride; /* device; NULL if no signing */
     const uint8_t *epk_table; size_t epk_stride; /* device; NULL if no verifying */
     int32_t n_keys;
-    int32_t max_batch;      /* requests per slot */
+    int32_t max_batch;      /* requests per slot (0 = 4096; at most CUMAYO_MAX_BATCH) */
     int32_t n_slots;        /* batches in flight */
     size_t slot_msg_bytes;  /* message bytes per slot */
     int64_t max_queue;      /* pending requests beyond which submit refuses (0 = no limit) */

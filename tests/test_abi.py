@@ -101,3 +101,55 @@ def test_launch_variants_and_config(version):
     assert got.as_dict() == alt.as_dict()
     assert lib.cumayo_set_launch_config(ctypes.byref(cfg)) == 0
     assert "cumayo_autotune" in EXPORTS
+
+
+def _max_batch():
+    m = re.search(r"#define\s+CUMAYO_MAX_BATCH\s+(\d+)u", open(HEADER).read())
+    assert m, "CUMAYO_MAX_BATCH missing from the header"
+    return int(m.group(1))
+
+
+@pytest.mark.parametrize("version", [1, 2, 3, 5])
+def test_batch_entry_points_refuse_bad_arguments(version):
+    """Argument checks that return before any work is enqueued (so they run
+    without a GPU): an empty batch is a no-op (0), a batch above
+    CUMAYO_MAX_BATCH and a null pointer are refused with hipErrorInvalidValue
+    (1), also by the void calls of the reference API, which record it for
+    cumayo_last_error. The pointer 4096 is a stand-in that is never read."""
+    from cumayo_amd import build
+    from cumayo_amd.engine import load_library
+    if not os.path.exists(build.lib_path(version)):
+        pytest.skip("library not built")
+    lib = load_library(version)
+    big = _max_batch() + 1
+    assert big - 1 == 2 ** 24 - 1
+    p = 4096
+    lib.cumayo_last_error()  # clear
+    sign_prep = lambda n, key=p: lib.cumayo_sign_prepared(p, key, p, p, p, p, n, p, None, 3)
+    verify_prep = lambda n, key=p: lib.cumayo_verify_prepared(p, key, p, p, p, p, p, n, p, None, 3)
+    sign_tab = lambda n, key=p: lib.cumayo_sign_table_stages(p, 64, key, p, p, p, p, n, p, None, 3)
+    verify_tab = lambda n, key=p: lib.cumayo_verify_table_stages(p, 64, key, p, p, p, p, p, n, p, None, 3)
+    wire_sign = lambda n, key=p: lib.cumayo_crypto_sign(p, key, p, p, p, p, p, n, p, None)
+    wire_open = lambda n, key=p: lib.cumayo_crypto_sign_open(p, key, p, p, p, p, p, p, p, n, p, None)
+    for name, call in (("sign_prepared", sign_prep), ("verify_prepared", verify_prep),
+                       ("sign_table_stages", sign_tab), ("verify_table_stages", verify_tab),
+                       ("crypto_sign", wire_sign), ("crypto_sign_open", wire_open)):
+        assert call(0) == 0, name
+        assert call(big) == 1, name
+        assert call(big - 1, key=None) == 1, name  # null key_idx
+    # a prepared table must be 16-byte aligned
+    assert lib.cumayo_sign_prepared(p + 4, p, p, p, p, p, 8, p, None, 3) == 1
+    # the comparison form (stage bit 4) is not available over prepared tables
+    assert lib.cumayo_sign_prepared(p, p, p, p, p, p, 8, p, None, 7) == 1
+    # key preparation: at most 65,535 keys per call, aligned destination
+    assert lib.cumayo_prepare_esk(p, 64, 0, p, None) == 0
+    assert lib.cumayo_prepare_esk(p, 64, 65536, p, None) == 1
+    assert lib.cumayo_prepare_epk(p, 64, 4, p + 8, None) == 1
+    # the reference API's void calls record the refusal
+    lib.cumayo_last_error()  # clear (the refusals above are recorded too)
+    assert lib.cumayo_last_error() == 0
+    lib.crypto_sign_expandsk(p, 1 << 20, big, None)
+    assert lib.cumayo_last_error() == 1
+    assert lib.cumayo_last_error() == 0  # reading clears it
+    lib.crypto_sign_verify(p, 32, 1 << 20, big, None)
+    assert lib.cumayo_last_error() == 1

@@ -217,3 +217,56 @@ def test_serve_backpressure_and_cancel(engine_factory):
             assert status == 3
         else:
             assert status == 0 and sig.tobytes() == batch_sigs[tag].tobytes()
+
+
+def test_serve_key_table_beyond_one_preparation_call(engine_factory):
+    """A server over 70,000 keys prepares its resident tables in chunks of
+    65,535 keys (the most one preparation call takes). Real key pairs sit on
+    both sides of the chunk edge (rows 65,534 .. 65,536) and at the ends, the
+    other rows are random bytes: every request signs and verifies exactly as
+    the batch entry points do on the same table rows."""
+    import torch
+    from cumayo_amd.serve import Server
+    v = 2
+    eng = engine_factory(v)
+    z = eng.sizes
+    dev = eng.device
+    nk = 70_000
+    rows = [0, 65_534, 65_535, 65_536, nk - 1]
+    esk_real, epk_real = eng.keypair_tables(len(rows), rand_index=77)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    esk = torch.randint(0, 256, (nk, z.esk_bytes), dtype=torch.uint8, device=dev, generator=gen)
+    epk = torch.randint(0, 256, (nk, z.epk_bytes), dtype=torch.uint8, device=dev, generator=gen)
+    at = torch.tensor(rows, device=dev)
+    esk[at] = esk_real
+    epk[at] = epk_real
+    keys = np.array(rows * 2 + [1, 65_537], np.int32)  # the last two are random rows
+    n = keys.size
+    rng = np.random.default_rng(8)
+    lens = rng.integers(0, 300, n).astype(np.int32)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    print(f"[serve] MAYO_{v} {nk} keys, n={n}", flush=True)
+    sigs = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+    eng.sign_table(esk, t(keys), t(flat), t(offs), t(lens), sigs, eng.sign_workspace(n))
+    res = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    eng.verify_table(epk, t(keys), t(flat), t(offs), t(lens), sigs, res, eng.verify_workspace(n))
+    torch.cuda.synchronize()
+    batch_sigs, batch_res = sigs.cpu().numpy(), res.cpu().numpy()
+    assert batch_res[:2 * len(rows)].tolist() == [0] * (2 * len(rows))
+    msg = lambda i: flat[offs[i]:offs[i] + lens[i]].tobytes()
+    reqs = [(i, 0, int(keys[i]), msg(i), None) for i in range(n)]
+    reqs += [(n + i, 1, int(keys[i]), msg(i), batch_sigs[i].tobytes()) for i in range(n)]
+    try:
+        with Server(eng, esk, epk, max_batch=8, n_slots=2) as srv:
+            assert srv.submit(reqs) == len(reqs)
+            got = srv.wait(range(2 * n))
+    finally:
+        del esk, epk
+        torch.cuda.empty_cache()
+    for i in range(n):
+        st, _, sig, _ = got[i]
+        assert st == 0 and sig.tobytes() == batch_sigs[i].tobytes(), (i, int(keys[i]))
+        assert got[n + i][0] == int(batch_res[i]), (i, int(keys[i]))
