@@ -33,50 +33,69 @@ __device__ __forceinline__ void expand_key(const uint8_t *key, uint32_t *rk, con
     }
 }
 
-// Table layout: T0 replicated 32 times, entry e of copy c at word
-// 32 e + c, and lane l reads copy l % 32, so the 32 lanes of a half-wave
-// always hit 32 different LDS banks whatever the indices. The S-box bytes of
-// the last round come from the same table (byte 2 of T0[x] is S[x]).
+// Table layout: T0 replicated 64 times, entry e of copy c at byte
+// 256 e + 4 c, and lane l reads copy l % 64, so the lanes of a half-wave
+// always hit different LDS banks whatever the indices. A lookup address is
+// then one v_perm_b32 of the state word and the lane's copy offset
+// (4 (l % 64)): the index byte goes to byte 1, the offset to byte 0. The
+// S-box bytes of the last round come from the same table (byte 2 of T0[x]
+// is S[x]).
 struct Replicated {
-    uint32_t T0[256 * 32];
+    static constexpr int COPIES = 64;
+    uint32_t T0[256 * COPIES];
 };
 
-__device__ __forceinline__ uint32_t rt(const Replicated &tb, uint32_t idx, uint32_t lane32) {
-    return tb.T0[(idx << 5) | lane32];
+__device__ __forceinline__ uint32_t copy_offset() { return (threadIdx.x & 63u) << 2; }
+
+// T0[byte B of s] from this lane's copy (off = copy_offset())
+template <int B>
+__device__ __forceinline__ uint32_t rt(const Replicated &tb, uint32_t s, uint32_t off) {
+    // v_perm_b32 selectors: 4..7 = bytes of s, 0..3 = bytes of off, 0x0C = zero
+    constexpr uint32_t sel = 0x0C0C0000u | ((4u + B) << 8);
+    return *(const uint32_t *)((const uint8_t *)tb.T0 + __builtin_amdgcn_perm(s, off, sel));
 }
 
 __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
-// One output column of a middle round: T0[a] ^ ror8(T0[b]) ^ ror16(T0[c]) ^ ror24(T0[d]) ^ k.
-__device__ __forceinline__ uint32_t round_col(const Replicated &tb, uint32_t l, uint32_t a, uint32_t b,
+// One output column of a middle round from state words a, b, c, d:
+// T0[a.b3] ^ ror8(T0[b.b2]) ^ ror16(T0[c.b1]) ^ ror24(T0[d.b0]) ^ k.
+__device__ __forceinline__ uint32_t round_col(const Replicated &tb, uint32_t off, uint32_t a, uint32_t b,
                                               uint32_t c, uint32_t d, uint32_t k) {
-    return xor3(xor3(rt(tb, a, l), ror(rt(tb, b, l), 8), ror(rt(tb, c, l), 16)), ror(rt(tb, d, l), 24), k);
+    return xor3(xor3(rt<3>(tb, a, off), ror(rt<2>(tb, b, off), 8), ror(rt<1>(tb, c, off), 16)),
+                ror(rt<0>(tb, d, off), 24), k);
+}
+
+// Last-round column: the S-box bytes (byte 2 of the T0 entries) of a.b3,
+// b.b2, c.b1, d.b0 in bytes 3, 2, 1, 0, plus k.
+__device__ __forceinline__ uint32_t last_col(const Replicated &tb, uint32_t off, uint32_t a, uint32_t b,
+                                             uint32_t c, uint32_t d, uint32_t k) {
+    const uint32_t hi = __builtin_amdgcn_perm(rt<3>(tb, a, off), rt<2>(tb, b, off), 0x06020C0Cu);
+    const uint32_t lo = __builtin_amdgcn_perm(rt<1>(tb, c, off), rt<0>(tb, d, off), 0x0C0C0602u);
+    return (hi | lo) ^ k;
 }
 
 // Encrypt the CTR block for block index blk: input 0^64 || BE64(blk).
 // Output words are big-endian (byte 0 in bits 31..24 of o[0]).
 __device__ __forceinline__ void ctr_block(const uint32_t *rk, const Replicated &tb, uint64_t blk,
                                           uint32_t o[4]) {
-    const uint32_t l = threadIdx.x & 31;
+    const uint32_t off = copy_offset();
     uint32_t s0 = rk[0], s1 = rk[1];
     uint32_t s2 = (uint32_t)(blk >> 32) ^ rk[2];
     uint32_t s3 = (uint32_t)blk ^ rk[3];
 #pragma unroll
     for (int r = 1; r < 10; ++r) {
-        const uint32_t t0 = round_col(tb, l, s0 >> 24, (s1 >> 16) & 0xFF, (s2 >> 8) & 0xFF, s3 & 0xFF, rk[4 * r]);
-        const uint32_t t1 = round_col(tb, l, s1 >> 24, (s2 >> 16) & 0xFF, (s3 >> 8) & 0xFF, s0 & 0xFF, rk[4 * r + 1]);
-        const uint32_t t2 = round_col(tb, l, s2 >> 24, (s3 >> 16) & 0xFF, (s0 >> 8) & 0xFF, s1 & 0xFF, rk[4 * r + 2]);
-        const uint32_t t3 = round_col(tb, l, s3 >> 24, (s0 >> 16) & 0xFF, (s1 >> 8) & 0xFF, s2 & 0xFF, rk[4 * r + 3]);
+        const uint32_t t0 = round_col(tb, off, s0, s1, s2, s3, rk[4 * r]);
+        const uint32_t t1 = round_col(tb, off, s1, s2, s3, s0, rk[4 * r + 1]);
+        const uint32_t t2 = round_col(tb, off, s2, s3, s0, s1, rk[4 * r + 2]);
+        const uint32_t t3 = round_col(tb, off, s3, s0, s1, s2, rk[4 * r + 3]);
         s0 = t0; s1 = t1; s2 = t2; s3 = t3;
     }
-    // last round: S-box bytes (bits 16..23 of T0) placed by byte position
-    auto sb = [&](uint32_t idx) { return (rt(tb, idx, l) >> 16) & 0xFFu; };
-    o[0] = (sb(s0 >> 24) << 24 | sb((s1 >> 16) & 0xFF) << 16 | sb((s2 >> 8) & 0xFF) << 8 | sb(s3 & 0xFF)) ^ rk[40];
-    o[1] = (sb(s1 >> 24) << 24 | sb((s2 >> 16) & 0xFF) << 16 | sb((s3 >> 8) & 0xFF) << 8 | sb(s0 & 0xFF)) ^ rk[41];
-    o[2] = (sb(s2 >> 24) << 24 | sb((s3 >> 16) & 0xFF) << 16 | sb((s0 >> 8) & 0xFF) << 8 | sb(s1 & 0xFF)) ^ rk[42];
-    o[3] = (sb(s3 >> 24) << 24 | sb((s0 >> 16) & 0xFF) << 16 | sb((s1 >> 8) & 0xFF) << 8 | sb(s2 & 0xFF)) ^ rk[43];
+    o[0] = last_col(tb, off, s0, s1, s2, s3, rk[40]);
+    o[1] = last_col(tb, off, s1, s2, s3, s0, rk[41]);
+    o[2] = last_col(tb, off, s2, s3, s0, s1, rk[42]);
+    o[3] = last_col(tb, off, s3, s0, s1, s2, rk[43]);
 }
 
 }  // namespace aes

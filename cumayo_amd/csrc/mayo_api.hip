@@ -143,7 +143,7 @@ void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) 
     const size_t p1_off = SK + O_BYTES;
     k_seed<<<blocks64(n), 64, 0, stream>>>(pool, pitch, n, 0, 0, 0, p1_off, SK);
     note_launch();
-    k_aes_expand<<<(unsigned)n, WG, 0, stream>>>(pool, pitch, n, p1_off, p1_off);
+    launch_aes_expand(pool, pitch, n, p1_off, p1_off, stream);
     note_launch();
     wv::k_expand_L_wave<<<(unsigned)n, 64, 0, stream>>>(pool, pitch, n);
     note_launch();
@@ -216,7 +216,7 @@ void crypto_sign_keypair(uint8_t *pool, size_t pitch, size_t batch_size, hipStre
     k_seed<<<blocks64(batch_size), 64, 0, stream>>>(pool, pitch, batch_size, 1, rand_index, 0,
                                                     pk_off, pk_off + 16);
     note_launch();
-    k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
+    launch_aes_expand(pool, pitch, batch_size, pk_off, p_off, stream);
     note_launch();
     wv::k_keypair_P3_wave<<<(unsigned)batch_size, 64, 0, stream>>>(pool, pitch, batch_size, pk_off, p_off);
     note_launch();
@@ -251,7 +251,7 @@ void crypto_sign_expandpk_HtD(uint8_t *cpk, uint8_t *pool, size_t pitch, size_t 
 void crypto_sign_expandpk(uint8_t *pool, size_t pitch, size_t batch_size, hipStream_t stream) {
     if (!batch_size || !batch_ok(batch_size)) return;
     ensure_tables();
-    k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, P_BYTES - 16, 0);
+    launch_aes_expand(pool, pitch, batch_size, P_BYTES - 16, 0, stream);
     note_launch();
 }
 
@@ -330,7 +330,7 @@ void crypto_sign_verifyWithepk(uint8_t *pool, size_t Mlen, size_t pitch, size_t 
                                hipStream_t stream) {
     if (!batch_size || !batch_ok(batch_size)) return;
     ensure_tables();
-    k_aes_expand<<<(unsigned)batch_size, WG, 0, stream>>>(pool, pitch, batch_size, P_BYTES - 16, 0);
+    launch_aes_expand(pool, pitch, batch_size, P_BYTES - 16, 0, stream);
     note_launch();
     launch_verify_pitched(pool, pitch, batch_size, Mlen, stream);
 }

@@ -368,43 +368,90 @@ __global__ void __launch_bounds__(64) k_verify_hash(Acc acc, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// AES-128-CTR expansion: one workgroup per key
+// AES-128-CTR expansion
 // ---------------------------------------------------------------------------
 
 __constant__ uint32_t c_aes_T0[256];
 __constant__ uint32_t c_aes_S[256];
 
-// out[0, P_BYTES) = keystream of key (aes128_ctr.cu:140-150 counter format).
-// key and out may overlap: the key is staged in LDS before any store.
-__global__ void __launch_bounds__(WG) k_aes_expand(uint8_t *pool, size_t pitch, size_t n,
-                                                   size_t key_off, size_t out_off) {
+// out[0, P_BYTES) = keystream of key (aes128_ctr.cu:140-150 counter format)
+// for n pool rows: row r's 16-byte key at key_off, its output at out_off.
+// A workgroup stages the replicated round table once and takes groups of
+// consecutive rows (grid-stride), KW waves per row (a power of two up to 16,
+// chosen by the launch so that small batches still fill the GPU): the
+// row's round keys are expanded into LDS, then read into SGPRs by the waves
+// that encrypt that row's counter blocks, so they (and the first-round
+// lookups of the key-only state words) stay out of the block loop. A row's
+// key is read before any of that row's output is stored, so the two may
+// overlap. The 64 KB table allows two workgroups per CU; at 1,024 threads
+// each that is 8 waves per SIMD.
+constexpr int AES_WG = 1024;
+constexpr int AES_KEYS = AES_WG / 64;  // rows per group at one wave per row
+template <int KW>
+__global__ void __launch_bounds__(AES_WG, 8) k_aes_expand(uint8_t *pool, size_t pitch, size_t n,
+                                                          size_t key_off, size_t out_off) {
+    constexpr int kw = KW;
     __shared__ aes::Replicated tb;
     __shared__ uint32_t S[256];
-    __shared__ uint32_t rk[44];
-    __shared__ uint8_t key[16];
-    const int tid = threadIdx.x;
-    uint8_t *row = pool + (size_t)blockIdx.x * pitch;
-    for (int e = tid; e < 256 * 32; e += WG) tb.T0[e] = c_aes_T0[e >> 5];
-    for (int e = tid; e < 256; e += WG) S[e] = c_aes_S[e];
-    if (tid < 16) key[tid] = row[key_off + tid];
+    __shared__ uint32_t rk[AES_KEYS][44];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int keys = AES_KEYS / kw;  // rows per group
+    const int row = wave / kw, sub = wave % kw;
+    for (int e = tid; e < 256 * aes::Replicated::COPIES; e += AES_WG) tb.T0[e] = c_aes_T0[e / aes::Replicated::COPIES];
+    for (int e = tid; e < 256; e += AES_WG) S[e] = c_aes_S[e];
     __syncthreads();
-    if (tid == 0) aes::expand_key(key, rk, S);
-    __syncthreads();
-    uint8_t *out = row + out_off;
-    const bool aligned = (((uintptr_t)out) & 3) == 0;
-    for (int blk = tid; blk < AES_BLOCKS; blk += WG) {
-        uint32_t o[4];
-        aes::ctr_block(rk, tb, (uint64_t)blk, o);
-        uint8_t *p = out + 16 * (size_t)blk;
-        int nb = P_BYTES - 16 * blk;
-        if (nb >= 16 && aligned) {
-            uint32_t *q = (uint32_t *)p;
-            q[0] = __builtin_bswap32(o[0]); q[1] = __builtin_bswap32(o[1]);
-            q[2] = __builtin_bswap32(o[2]); q[3] = __builtin_bswap32(o[3]);
-        } else {
-            if (nb > 16) nb = 16;
-            for (int b = 0; b < nb; ++b) p[b] = (uint8_t)(o[b >> 2] >> (24 - 8 * (b & 3)));
+    const size_t ngroups = (n + keys - 1) / keys;
+    for (size_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const size_t k0 = g * keys;
+        const int nk = n - k0 < (size_t)keys ? (int)(n - k0) : keys;
+        if (tid < nk) {
+            const uint8_t *src = pool + (k0 + tid) * pitch + key_off;
+            uint8_t key[16];
+#pragma unroll
+            for (int b = 0; b < 16; ++b) key[b] = src[b];
+            aes::expand_key(key, rk[tid], S);
         }
+        __syncthreads();
+        if (row < nk) {
+            uint32_t rks[44];
+#pragma unroll
+            for (int i = 0; i < 44; ++i) rks[i] = __builtin_amdgcn_readfirstlane(rk[row][i]);
+            uint8_t *out = pool + (k0 + row) * pitch + out_off;
+            const bool aligned = (((uintptr_t)out) & 3) == 0;
+            for (int blk = sub * 64 + lane; blk < AES_BLOCKS; blk += 64 * kw) {
+                uint32_t o[4];
+                aes::ctr_block(rks, tb, (uint64_t)blk, o);
+                uint8_t *p = out + 16 * (size_t)blk;
+                int nb = P_BYTES - 16 * blk;
+                if (nb >= 16 && aligned) {
+                    uint32_t *q = (uint32_t *)p;
+                    q[0] = __builtin_bswap32(o[0]); q[1] = __builtin_bswap32(o[1]);
+                    q[2] = __builtin_bswap32(o[2]); q[3] = __builtin_bswap32(o[3]);
+                } else {
+                    if (nb > 16) nb = 16;
+                    for (int b = 0; b < nb; ++b) p[b] = (uint8_t)(o[b >> 2] >> (24 - 8 * (b & 3)));
+                }
+            }
+        }
+        __syncthreads();  // rk is rewritten for the next group
+    }
+}
+// Launch of k_aes_expand over n rows: waves per row kw = the power of two
+// (1 .. 16) that brings n kw to about 8,192 waves (32 per CU), and at most
+// 2,048 workgroups, each looping over its groups of rows.
+inline void launch_aes_expand(uint8_t *pool, size_t pitch, size_t n, size_t key_off, size_t out_off,
+                              hipStream_t stream) {
+    int kw = 1;
+    while (kw < AES_KEYS && n * (size_t)kw < 8192) kw *= 2;
+    const size_t groups = (n + (size_t)(AES_KEYS / kw) - 1) / (size_t)(AES_KEYS / kw);
+    const dim3 grid((unsigned)(groups < 2048 ? groups : 2048));
+    switch (kw) {
+        case 1: k_aes_expand<1><<<grid, AES_WG, 0, stream>>>(pool, pitch, n, key_off, out_off); break;
+        case 2: k_aes_expand<2><<<grid, AES_WG, 0, stream>>>(pool, pitch, n, key_off, out_off); break;
+        case 4: k_aes_expand<4><<<grid, AES_WG, 0, stream>>>(pool, pitch, n, key_off, out_off); break;
+        case 8: k_aes_expand<8><<<grid, AES_WG, 0, stream>>>(pool, pitch, n, key_off, out_off); break;
+        default: k_aes_expand<16><<<grid, AES_WG, 0, stream>>>(pool, pitch, n, key_off, out_off); break;
     }
 }
 
