@@ -83,6 +83,8 @@ class TuneReport(ctypes.Structure):
 
 
 LAUNCH_FIELDS = {"sign_prep": 0, "sign_solve": 1, "verify": 2, "order": 3, "sign_chunks": 4}
+# keys per call of cumayo_prepare_esk / cumayo_prepare_epk
+PREPARE_KEYS_MAX = 65535
 
 # (name, argtypes) of every symbol the header declares
 _VP = ctypes.c_void_p
@@ -455,9 +457,12 @@ class Engine:
                 continue
             nk = tab.shape[0]
             prep = torch.empty((nk, int(nbytes)), dtype=torch.uint8, device=self.device)
-            err = fn(tab.data_ptr(), tab.stride(0), nk, prep.data_ptr(), st)
-            if err:
-                raise RuntimeError(f"cumayo_prepare_* failed: HIP error {err}")
+            # one call takes at most PREPARE_KEYS_MAX keys
+            for k0 in range(0, nk, PREPARE_KEYS_MAX):
+                k = min(PREPARE_KEYS_MAX, nk - k0)
+                err = fn(tab[k0].data_ptr(), tab.stride(0), k, prep[k0].data_ptr(), st)
+                if err:
+                    raise RuntimeError(f"cumayo_prepare_* failed: HIP error {err}")
             out.append(prep)
         return tuple(out)
 

@@ -256,6 +256,13 @@ def test_serve_key_table_beyond_one_preparation_call(engine_factory):
     torch.cuda.synchronize()
     batch_sigs, batch_res = sigs.cpu().numpy(), res.cpu().numpy()
     assert batch_res[:2 * len(rows)].tolist() == [0] * (2 * len(rows))
+    # the Python wrapper prepares the whole table in chunks too: its rows
+    # equal those prepared from the real key rows alone
+    pesk, pepk = eng.prepare_tables(esk, epk)
+    pesk_real, pepk_real = eng.prepare_tables(esk_real, epk_real)
+    torch.cuda.synchronize()
+    assert torch.equal(pesk[at], pesk_real) and torch.equal(pepk[at], pepk_real)
+    del pesk, pepk
     msg = lambda i: flat[offs[i]:offs[i] + lens[i]].tobytes()
     reqs = [(i, 0, int(keys[i]), msg(i), None) for i in range(n)]
     reqs += [(n + i, 1, int(keys[i]), msg(i), batch_sigs[i].tobytes()) for i in range(n)]
