@@ -313,12 +313,15 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                 for (int b = 0; b < 4; ++b) Tb[tt][j][b] = 0;
         }
         // one column: every row of the chunk takes its entry at c, fetched
-        // during the previous column; off[tt] is the next column to fetch
-        EntryLoad<KL> ld[RT];
-        auto fetch = [&]() {
+        // into load set S during an earlier column; off[tt] is the next
+        // column to fetch
+        EntryLoad<KL> ld[2][RT];
+        using Set0 = std::integral_constant<int, 0>;
+        using Set1 = std::integral_constant<int, 1>;
+        auto fetch = [&](auto S) {
 #pragma unroll
             for (int tt = 0; tt < RT; ++tt) {
-                ld[tt].fetch(key, off[tt], nv);
+                ld[decltype(S)::value][tt].fetch(key, off[tt], nv);
                 off[tt] += stride[tt];
             }
         };
@@ -328,7 +331,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
 #pragma unroll
         for (int tt = 0; tt < RT; ++tt) tt_start[tt] = v0 + tt * SL < NVR ? start_of(v0 + tt * SL) : NP;
         // groups 0 .. NTT-1 take column c
-        auto column = [&](auto NT, int c, bool masked, bool more) {
+        auto column = [&](auto NT, auto S, int c, bool masked, bool more) {
             constexpr int NTT = decltype(NT)::value;
             const uint64_t sc = uni64(scol[c]);
             // masks from the 32-bit halves: one scalar bit-field extract each
@@ -336,10 +339,10 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
             uint32_t x[RT];
 #pragma unroll
             for (int tt = 0; tt < RT; ++tt) {
-                x[tt] = ld[tt].word(nv);
+                x[tt] = ld[decltype(S)::value][tt].word(nv);
                 if (masked) x[tt] = c >= st[tt] ? x[tt] : 0u;
             }
-            if (more) fetch();
+            if (more) fetch(S);
 #pragma unroll
             for (int j = 0; j < K; ++j)
 #pragma unroll
@@ -352,19 +355,35 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         };
         using All = std::integral_constant<int, RT>;
         // columns [c0, cend), masked below cm, in three loops: only group 0
-        // started (RT = 2), all groups masked, unmasked
+        // started (RT = 2), all groups masked (loads one column ahead), and
+        // unmasked, the long one. With aligned entries (one register per
+        // load) that one runs in pairs of columns over the two load sets,
+        // loading two columns ahead, which covers more of the memory latency
+        // when the key rows come from HBM rather than L2; unaligned entries
+        // (three registers per load) stay one column ahead.
         auto segment = [&](int c0, int cm, int cend) {
-            if (c0 < cend) fetch();
+            if (c0 < cend) fetch(Set0{});
             int c = c0;
             if constexpr (RT == 2) {
                 const int c1 = tt_start[1] < cm ? tt_start[1] : cm;
 #pragma unroll 1
-                for (; c < c1; ++c) column(std::integral_constant<int, 1>{}, c, true, c + 1 < cend);
+                for (; c < c1; ++c) column(std::integral_constant<int, 1>{}, Set0{}, c, true, c + 1 < cend);
             }
 #pragma unroll 1
-            for (; c < cm; ++c) column(All{}, c, true, c + 1 < cend);
+            for (; c < cm; ++c) column(All{}, Set0{}, c, true, c + 1 < cend);
+            if constexpr (KL::ALIGNED) {
+                // set 0 holds column c; set 1 takes c + 1
+                if (c + 1 < cend) fetch(Set1{});
 #pragma unroll 1
-            for (; c < cend; ++c) column(All{}, c, false, c + 1 < cend);
+                for (; c + 1 < cend; c += 2) {
+                    column(All{}, Set0{}, c, false, c + 2 < cend);
+                    column(All{}, Set1{}, c + 1, false, c + 3 < cend);
+                }
+                if (c < cend) column(All{}, Set0{}, c, false, false);
+            } else {
+#pragma unroll 1
+                for (; c < cend; ++c) column(All{}, Set0{}, c, false, c + 1 < cend);
+            }
         };
         // columns < V (P1 and L entries)
         if (c_lo < V) segment(c_lo, c_free < V ? c_free : V, V);
