@@ -315,9 +315,15 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         // one column: every row of the chunk takes its entry at c, fetched
         // into load set S during an earlier column; off[tt] is the next
         // column to fetch
-        EntryLoad<KL> ld[2][RT];
+        // load sets: over aligned entries (one register per load) three for
+        // m <= 80 and two for the larger sets, whose verification kernel
+        // spills with three; one over unaligned reference-layout entries
+        // (three registers per load)
+        constexpr int NSET = KL::ALIGNED ? (M <= 80 ? 3 : 2) : 1;
+        EntryLoad<KL> ld[NSET][RT];
         using Set0 = std::integral_constant<int, 0>;
-        using Set1 = std::integral_constant<int, 1>;
+        using Set1 = std::integral_constant<int, (NSET > 1 ? 1 : 0)>;
+        using Set2 = std::integral_constant<int, (NSET > 2 ? 2 : 0)>;
         auto fetch = [&](auto S) {
 #pragma unroll
             for (int tt = 0; tt < RT; ++tt) {
@@ -356,11 +362,12 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         using All = std::integral_constant<int, RT>;
         // columns [c0, cend), masked below cm, in three loops: only group 0
         // started (RT = 2), all groups masked (loads one column ahead), and
-        // unmasked, the long one. With aligned entries (one register per
-        // load) that one runs in pairs of columns over the two load sets,
-        // loading two columns ahead, which covers more of the memory latency
-        // when the key rows come from HBM rather than L2; unaligned entries
-        // (three registers per load) stay one column ahead.
+        // unmasked, the long one. Over aligned entries that one runs in runs
+        // of NSET columns over the NSET load sets, loading NSET columns ahead
+        // (set s holds column c + s, then takes c + s + NSET). Loading 1, 2
+        // and 3 columns ahead took MAYO_1 stage 2 (64 shared keys) from 5.58
+        // to 5.41 to 5.36 ms and MAYO_2 over 65,536 distinct keys from 3.01
+        // to 2.71 to 2.66 ms; four ahead spills the verification kernel.
         auto segment = [&](int c0, int cm, int cend) {
             if (c0 < cend) fetch(Set0{});
             int c = c0;
@@ -371,8 +378,20 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
             }
 #pragma unroll 1
             for (; c < cm; ++c) column(All{}, Set0{}, c, true, c + 1 < cend);
-            if constexpr (KL::ALIGNED) {
-                // set 0 holds column c; set 1 takes c + 1
+            if constexpr (NSET == 3) {
+                // set 0 holds column c; sets 1 and 2 take c + 1 and c + 2
+                if (c + 1 < cend) fetch(Set1{});
+                if (c + 2 < cend) fetch(Set2{});
+#pragma unroll 1
+                for (; c + 2 < cend; c += 3) {
+                    column(All{}, Set0{}, c, false, c + 3 < cend);
+                    column(All{}, Set1{}, c + 1, false, c + 4 < cend);
+                    column(All{}, Set2{}, c + 2, false, c + 5 < cend);
+                }
+                // fewer than three columns left, already loaded
+                if (c < cend) column(All{}, Set0{}, c, false, false);
+                if (c + 1 < cend) column(All{}, Set1{}, c + 1, false, false);
+            } else if constexpr (NSET == 2) {
                 if (c + 1 < cend) fetch(Set1{});
 #pragma unroll 1
                 for (; c + 1 < cend; c += 2) {
