@@ -1148,8 +1148,8 @@ __device__ __forceinline__ void load_ocol(const uint8_t *Onib, uint32_t *ocol) {
 }
 
 // row_done(r, T) is called for each valid row of the lane with T[x] = T_x[r]
-// (this lane's word w).
-template <bool SYM, class RowDone>
+// (this lane's word w). Entry words are loaded PF columns ahead.
+template <bool SYM, int PF, class RowDone>
 __device__ __forceinline__ void p1_times_O(const uint8_t *P1, const uint32_t *ocol, RowDone &&row_done) {
     const int lane = threadIdx.x;
     const int w = lane % MW, slot = lane / MW;
@@ -1184,7 +1184,6 @@ __device__ __forceinline__ void p1_times_O(const uint8_t *P1, const uint32_t *oc
             return (SYM && c < rr[tt]) ? lo[tt] + lb : up[tt] + ub;
         };
         // the loads of the next PF columns are in flight during each column
-        constexpr int PF = 1;
         wv::EntryLoad<RefKeys> ld[PF][KRT];
         const int cstart = SYM ? 0 : r0;
 #pragma unroll
@@ -1237,6 +1236,13 @@ __device__ __forceinline__ void p1_times_O(const uint8_t *P1, const uint32_t *oc
     }
 }
 
+// Columns k_expand_L_wave loads ahead: 3 for MAYO_1 and 2 for MAYO_3 (65,536
+// expansions: 16.2 -> 15.2 ms and 43.4 -> 41.3 ms); 1 for MAYO_2, whose
+// kernel drops from 5 to 4 waves per SIMD with more (11.0 -> 13.4 ms), and
+// for MAYO_5, which spills with more. The keypair kernel loads one ahead:
+// more would cost it occupancy.
+constexpr int EXPAND_PF = KRT == 1 || M > 108 ? 1 : (M <= 80 ? 3 : 2);
+
 // expandsk: L = (P1 + P1^T) O + P2 in place over P2.
 __global__ void __launch_bounds__(64, 3) k_expand_L_wave(uint8_t *pool, size_t pitch, size_t n) {
     __shared__ uint32_t ocol[V * OWD];
@@ -1246,7 +1252,7 @@ __global__ void __launch_bounds__(64, 3) k_expand_L_wave(uint8_t *pool, size_t p
     load_ocol(esk + SK, ocol);
     __syncthreads();
     const int w = threadIdx.x % MW;
-    p1_times_O<true>(P1, ocol, [&](int r, const uint32_t (&T)[O]) {
+    p1_times_O<true, EXPAND_PF>(P1, ocol, [&](int r, const uint32_t (&T)[O]) {
 #pragma unroll
         for (int xo = 0; xo < O; ++xo) {
             uint8_t *e = L + (size_t)(r * O + xo) * MB;
@@ -1271,7 +1277,7 @@ __global__ void __launch_bounds__(64, 2) k_keypair_P3_wave(uint8_t *pool, size_t
     for (int e = threadIdx.x; e < P3_ENTRIES * MW; e += 64) P3acc[e] = 0;
     __syncthreads();
     const int w = threadIdx.x % MW;
-    p1_times_O<false>(P1, ocol, [&](int r, const uint32_t (&T)[O]) {
+    p1_times_O<false, 1>(P1, ocol, [&](int r, const uint32_t (&T)[O]) {
         uint32_t om[O][4];  // bit masks of O[r][x]
 #pragma unroll
         for (int x = 0; x < O; ++x) {
