@@ -155,7 +155,7 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
-    wv::k_sign_wave<PitchedSign><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+    wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }
 
@@ -163,7 +163,7 @@ void launch_verify_pitched(uint8_t *pool, size_t pitch, size_t n, size_t mlen, h
     if (!n || !batch_ok(n)) return;
     const size_t msg_off = EPK_PAD, sig_off = EPK_PAD + A128(mlen), res_off = sig_off + SIG_PAD;
     PitchedVerify acc{pool, pitch, msg_off, sig_off, res_off, (uint32_t)mlen};
-    wv::k_verify_wave<PitchedVerify><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+    wv::k_verify_wave<PitchedVerify, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }
 

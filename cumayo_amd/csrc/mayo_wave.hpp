@@ -147,16 +147,32 @@ struct Shared {
 // every entry word stored planar (gf16::planar), which makes the per-row
 // multiplications by a of the quadratic form a third as costly.
 constexpr int MBP = (MB + 3) & ~3;
+//
+// AHEAD: the columns the quad-form column loop loads ahead. Loading 1, 2 and
+// 3 ahead took MAYO_1 stage 2 over prepared keys (64 shared) from 5.58 to
+// 5.41 to 5.36 ms and MAYO_2 over 65,536 distinct keys from 3.01 to 2.71 to
+// 2.66 ms; the larger sets' verification kernel spills with 3, and with 4
+// every set's does. A reference-layout load takes three registers, so those
+// rows load one ahead, except in pitched pools, where every item reads its
+// own key row from HBM: there MAYO_2, whose kernels have registers to spare,
+// loads three ahead (drop-in signing and verification over 65,536 distinct
+// keys: 4.53 -> 4.67 M/s and 9.36 -> 9.43 M/s; the same depth made tables of
+// 64 shared keys 4-8% slower).
 struct RefKeys {
     static constexpr bool ALIGNED = false;
     static constexpr bool PLANAR = false;
     static constexpr int EB = MB;
+    static constexpr int AHEAD = 1;
     static constexpr uint32_t ESK_P1 = SK + O_BYTES, ESK_L = ESK_P1 + P1_BYTES;
     static constexpr uint32_t EPK_P1 = 0, EPK_P2 = P1_BYTES, EPK_P3 = P1_BYTES + P2_BYTES;
+};
+struct PitchedRefKeys : RefKeys {
+    static constexpr int AHEAD = M <= 64 ? 3 : 1;
 };
 struct PrepKeys {
     static constexpr bool ALIGNED = true;
     static constexpr bool PLANAR = true;
+    static constexpr int AHEAD = M <= 80 ? 3 : 2;
     static constexpr int EB = MBP;
     static constexpr uint32_t ESK_P1 = (SK + O_BYTES + 15) / 16 * 16;
     static constexpr uint32_t ESK_L = ESK_P1 + P1_ENTRIES * MBP;
@@ -315,11 +331,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         // one column: every row of the chunk takes its entry at c, fetched
         // into load set S during an earlier column; off[tt] is the next
         // column to fetch
-        // load sets: over aligned entries (one register per load) three for
-        // m <= 80 and two for the larger sets, whose verification kernel
-        // spills with three; one over unaligned reference-layout entries
-        // (three registers per load)
-        constexpr int NSET = KL::ALIGNED ? (M <= 80 ? 3 : 2) : 1;
+        constexpr int NSET = KL::AHEAD;  // load sets
         EntryLoad<KL> ld[NSET][RT];
         using Set0 = std::integral_constant<int, 0>;
         using Set1 = std::integral_constant<int, (NSET > 1 ? 1 : 0)>;
@@ -362,12 +374,9 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
         using All = std::integral_constant<int, RT>;
         // columns [c0, cend), masked below cm, in three loops: only group 0
         // started (RT = 2), all groups masked (loads one column ahead), and
-        // unmasked, the long one. Over aligned entries that one runs in runs
-        // of NSET columns over the NSET load sets, loading NSET columns ahead
-        // (set s holds column c + s, then takes c + s + NSET). Loading 1, 2
-        // and 3 columns ahead took MAYO_1 stage 2 (64 shared keys) from 5.58
-        // to 5.41 to 5.36 ms and MAYO_2 over 65,536 distinct keys from 3.01
-        // to 2.71 to 2.66 ms; four ahead spills the verification kernel.
+        // unmasked, the long one, in runs of NSET = KL::AHEAD columns over the
+        // NSET load sets, loading NSET columns ahead (set s holds column c + s,
+        // then takes c + s + NSET).
         auto segment = [&](int c0, int cm, int cend) {
             if (c0 < cend) fetch(Set0{});
             int c = c0;
