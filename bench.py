@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.00 and x1.00 by the
 # same counters over a 256 MiB copy) on the default workload; the run is
 # summarised in profiles/sign_verify_mayo1_65536x64keys.md
-STAGE2_TRAFFIC = {(1, 65536, 64): 568_236_267}
+STAGE2_TRAFFIC = {(1, 65536, 64): 568_312_940}
 
 
 def make_workload(n: int, nkeys: int, lo: int, hi: int, tag: str = "cfg2"):
