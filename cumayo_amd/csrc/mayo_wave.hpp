@@ -980,8 +980,13 @@ __device__ __forceinline__ void derive_vr(Shared &S, const SignItem &it, int c) 
 
 // The whole signing loop in one wave (sign.cu:244-478), for the reference
 // pool. Attempts below fail_first report a singular system (test hook).
+// Waves per SIMD of the whole-loop kernel. MAYO_3's spills 182 VGPRs at 3
+// waves and 55 at 2, and signs 6% faster at 2 (0.90 -> 0.95 M/s over 32,768
+// pitched distinct keys). MAYO_1's spills 105 at 3 and none at 2, yet signs
+// 9% slower at 2, so it keeps 3.
+constexpr int SIGN_WAVE_OCC = M > 80 && M <= 108 ? 2 : QF_OCC;
 template <class Acc, class KL = RefKeys>
-__global__ void __launch_bounds__(64, QF_OCC) k_sign_wave(Acc acc, size_t n, int fail_first = 0) {
+__global__ void __launch_bounds__(64, SIGN_WAVE_OCC) k_sign_wave(Acc acc, size_t n, int fail_first = 0) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
