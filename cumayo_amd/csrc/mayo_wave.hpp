@@ -147,7 +147,7 @@ struct Shared {
 // every entry word stored planar (gf16::planar), which makes the per-row
 // multiplications by a of the quadratic form a third as costly.
 constexpr int MBP = (MB + 3) & ~3;
-//
+
 // AHEAD: the columns the quad-form column loop loads ahead. Loading 1, 2 and
 // 3 ahead took MAYO_1 stage 2 over prepared keys (64 shared) from 5.58 to
 // 5.41 to 5.36 ms and MAYO_2 over 65,536 distinct keys from 3.01 to 2.71 to
@@ -156,8 +156,8 @@ constexpr int MBP = (MB + 3) & ~3;
 // rows load one ahead, except in pitched pools, where every item reads its
 // own key row from HBM: there MAYO_2, whose kernels have registers to spare,
 // loads three ahead (drop-in signing and verification over 65,536 distinct
-// keys: 4.53 -> 4.67 M/s and 9.36 -> 9.43 M/s; the same depth made tables of
-// 64 shared keys 4-8% slower).
+// keys: 4.50 -> 4.62 M/s and 9.27 -> 9.41 M/s; the same depth made tables of
+// 64 shared keys slower).
 struct RefKeys {
     static constexpr bool ALIGNED = false;
     static constexpr bool PLANAR = false;
@@ -978,13 +978,14 @@ __device__ __forceinline__ void derive_vr(Shared &S, const SignItem &it, int c) 
     keccak::wave_shake256(S.vr, VR_BYTES, S.hbuf, DIGEST + 2 * SK + 1);
 }
 
+// Waves per SIMD of the whole-loop kernel below. MAYO_3's spills 182 VGPRs
+// at 3 waves and 55 at 2, and signs 6% faster at 2 (0.90 -> 0.95 M/s over
+// 32,768 pitched distinct keys). MAYO_1's spills 105 at 3 and none at 2, yet
+// signs 9% slower at 2, so it keeps 3.
+constexpr int SIGN_WAVE_OCC = M > 80 && M <= 108 ? 2 : QF_OCC;
+
 // The whole signing loop in one wave (sign.cu:244-478), for the reference
 // pool. Attempts below fail_first report a singular system (test hook).
-// Waves per SIMD of the whole-loop kernel. MAYO_3's spills 182 VGPRs at 3
-// waves and 55 at 2, and signs 6% faster at 2 (0.90 -> 0.95 M/s over 32,768
-// pitched distinct keys). MAYO_1's spills 105 at 3 and none at 2, yet signs
-// 9% slower at 2, so it keeps 3.
-constexpr int SIGN_WAVE_OCC = M > 80 && M <= 108 ? 2 : QF_OCC;
 template <class Acc, class KL = RefKeys>
 __global__ void __launch_bounds__(64, SIGN_WAVE_OCC) k_sign_wave(Acc acc, size_t n, int fail_first = 0) {
     __shared__ Shared S;
@@ -1251,7 +1252,7 @@ __device__ __forceinline__ void p1_times_O(const uint8_t *P1, const uint32_t *oc
 }
 
 // Columns k_expand_L_wave loads ahead: 3 for MAYO_1 and 2 for MAYO_3 (65,536
-// expansions: 16.2 -> 15.2 ms and 43.4 -> 41.3 ms); 1 for MAYO_2, whose
+// expansions: 16.3 -> 15.3 ms and 43.7 -> 41.4 ms); 1 for MAYO_2, whose
 // kernel drops from 5 to 4 waves per SIMD with more (11.0 -> 13.4 ms), and
 // for MAYO_5, which spills with more. The keypair kernel loads one ahead:
 // more would cost it occupancy.
