@@ -201,6 +201,24 @@ def test_device_key_tables_match_oracle(engine_factory):
             assert epk[i].tobytes() == O.expandpk(v, opk)
 
 
+@pytest.mark.parametrize("n", (1024, 2048, 4096, 8192))
+def test_device_key_tables_every_aes_launch_shape(engine_factory, n):
+    """The AES-CTR expansion gives each key 8, 4, 2 and 1 waves at these
+    batch sizes (n times the waves per key is about 8,192) and so takes 2,
+    4, 8 and 16 keys per group: rows on both sides of every group size's
+    edges and the last row equal the oracle's expanded keys."""
+    v = 2
+    eng = engine_factory(v)
+    print(f"[parity] MAYO_{v} device key tables n={n}", flush=True)
+    esk, epk = eng.keypair_tables(n, rand_index=21)
+    rows = sorted({0, 1, 3, 4, 7, 8, 15, 16, n // 2 - 1, n // 2, n - 1})
+    esk, epk = esk[rows].cpu().numpy(), epk[rows].cpu().numpy()
+    for k, i in enumerate(rows):
+        opk, osk = O.keypair(v, 21, i)
+        assert esk[k].tobytes() == O.expandsk(v, osk), (n, i)
+        assert epk[k].tobytes() == O.expandpk(v, opk), (n, i)
+
+
 def _planar_words(entries):
     """Each little-endian dword of the entries (last axis, bytes) with bit
     4e + d moved to bit 8d + e: byte d then holds bit d of the 8 nibbles."""
