@@ -394,29 +394,33 @@ hipStream_t aux_stream() {
 }
 
 // Stage 2 of signing: attempt 0 in two kernels tuned for their register
-// needs (prep_k, solve_k), over the whole batch or, with parts = 2, over
-// two halves, the second on the helper stream after a fork from `stream`
-// and joined back before the retry pass; the kernels of one half then fill
-// the other half's tails. Then the rank failures (about 1 in 4,000) go
-// through the retry pass over the failure list.
+// needs (prep_k, solve_k), then the retry pass over the items whose attempt
+// met a singular system (about 1 in 4,000). Over the whole batch, or, with
+// parts = 2, over two halves, the second on the helper stream after a fork
+// from `stream` and joined back at the end: the kernels of one half fill the
+// other half's tails, and each half's retry pass (over the half's own
+// failure list) runs beside the other half's kernels.
 template <class KL, class PK, class SK>
 hipError_t launch_sign_stage2(PK prep_k, SK solve_k, const TableSign &acc, size_t n, int ordered, int fail_first,
                               int parts, hipStream_t stream) {
+    const unsigned retry_threads = 64 * wv::RNW;
+    // prep, solve and retry over the positions [base, base + m) on s
+    auto part = [&](size_t base, size_t m, hipStream_t s) {
+        hipLaunchKernelGGL(prep_k, dim3((unsigned)m), dim3(64), 0, s, acc, n, ordered, base);
+        hipLaunchKernelGGL(solve_k, dim3((unsigned)m), dim3(64), 0, s, acc, n, fail_first, ordered, base);
+        hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3((unsigned)(m < 256 ? m : 256)), dim3(retry_threads), 0, s,
+                           acc, n, fail_first, base);
+    };
     hipStream_t aux = parts > 1 ? aux_stream() : nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     if (parts > 1 && aux && hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess &&
         hipEventCreateWithFlags(&join, hipEventDisableTiming) == hipSuccess) {
-        // the list is emptied once, ahead of both halves' solve kernels
-        hipLaunchKernelGGL(wv::k_reset_list<TableSign>, dim3(1), dim3(64), 0, stream, acc);
         hipError_t e = hipEventRecord(fork, stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
         const size_t h = n / 2;
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(prep_k, dim3((unsigned)h), dim3(64), 0, stream, acc, n, ordered, (size_t)0, 0);
-            hipLaunchKernelGGL(solve_k, dim3((unsigned)h), dim3(64), 0, stream, acc, n, fail_first, ordered,
-                               (size_t)0);
-            hipLaunchKernelGGL(prep_k, dim3((unsigned)(n - h)), dim3(64), 0, aux, acc, n, ordered, h, 0);
-            hipLaunchKernelGGL(solve_k, dim3((unsigned)(n - h)), dim3(64), 0, aux, acc, n, fail_first, ordered, h);
+            part(0, h, stream);
+            part(h, n - h, aux);
             e = hipEventRecord(join, aux);
             if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
         }
@@ -425,12 +429,8 @@ hipError_t launch_sign_stage2(PK prep_k, SK solve_k, const TableSign &acc, size_
         if (e != hipSuccess) return e;
     } else {
         if (fork) (void)hipEventDestroy(fork);
-        hipLaunchKernelGGL(prep_k, dim3((unsigned)n), dim3(64), 0, stream, acc, n, ordered, (size_t)0, 1);
-        hipLaunchKernelGGL(solve_k, dim3((unsigned)n), dim3(64), 0, stream, acc, n, fail_first, ordered, (size_t)0);
+        part(0, n, stream);
     }
-    const unsigned retry_grid = (unsigned)(n < 256 ? n : 256);
-    hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3(retry_grid), dim3(64 * wv::RNW), 0, stream, acc, n,
-                       fail_first);
     return hipGetLastError();
 }
 

@@ -1034,17 +1034,19 @@ __device__ __noinline__ void retry_linear_algebra(Shared &S, int ctr, bool force
 // RNW waves per item, so that one item's attempt takes a fraction of a
 // single wave's time: the waves share the row chunks of the quadratic form,
 // wave 0 re-derives V || r and runs the linear algebra. Grid: any size; a
-// workgroup takes list entries blockIdx.x, + gridDim.x, ...
+// workgroup takes list entries blockIdx.x, + gridDim.x, ... of the failure
+// list kept at item `list` (the first item of the launch part that made it).
 constexpr int RNW_MAX = M <= 80 ? 8 : 4;  // larger sets: a whole SIMD's registers per wave
 constexpr int RNW = qf_chunks<O, false>() < RNW_MAX ? qf_chunks<O, false>() : RNW_MAX;
 template <class Acc, class KL>
-__global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, int fail_first = 0) {
+__global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, int fail_first = 0,
+                                                              size_t list = 0) {
     __shared__ Shared S;
     __shared__ int state;  // 0 next attempt, 1 solved, 2 attempts exhausted
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t count = *(const uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT);
+    const uint32_t count = *(const uint32_t *)(acc.scratch_of(list) + SCR_LIST_COUNT);
     for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
-        const uint32_t item = *(const uint32_t *)(acc.scratch_of(k) + SCR_LIST);
+        const uint32_t item = *(const uint32_t *)(acc.scratch_of(list + k) + SCR_LIST);
         SignItem it = acc(item);
         if (wave == 0) {
             for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
@@ -1073,24 +1075,17 @@ __global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, i
     }
 }
 
-// Empties the failure list of the solve kernels before the parts of a batch
-// whose stage 2 runs as several launches.
-template <class Acc>
-__global__ void k_reset_list(Acc acc) {
-    if (threadIdx.x == 0) *(uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT) = 0;
-}
-
 // Batch form, attempt 0, part 1: M, U and y, handed over through the
 // workspace. Register-heavy (the pair accumulators of the quadratic form).
 template <class Acc, class KL = RefKeys, int CODE = qf_code(QF_RT, QF_OCC)>
-__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0, size_t base = 0,
-                                                             int reset_list = 1) {
+__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0, size_t base = 0) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(item_of(acc, ordered, base));
-    // the failure list of the solve kernel starts empty (a launch over part
-    // of a batch leaves that to k_reset_list, ahead of every part)
-    if (reset_list && blockIdx.x == 0 && lane == 0) *(uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT) = 0;
+    // the failure list of this launch part's solve kernel starts empty; it
+    // lives in the part's own workspace slots (count in slot base, entries
+    // in slots base, base + 1, ...), so parts never share one
+    if (blockIdx.x == 0 && lane == 0) *(uint32_t *)(acc.scratch_of(base) + SCR_LIST_COUNT) = 0;
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     __syncthreads();
@@ -1129,8 +1124,8 @@ __global__ void __launch_bounds__(64, OCC) k_sign_solve(Acc acc, size_t n, int f
         // pending: listed for the retry pass
         if (lane == 0) {
             it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
-            const uint32_t k = atomicAdd((uint32_t *)(acc.scratch_of(0) + SCR_LIST_COUNT), 1u);
-            *(uint32_t *)(acc.scratch_of(k) + SCR_LIST) = (uint32_t)item;
+            const uint32_t k = atomicAdd((uint32_t *)(acc.scratch_of(base) + SCR_LIST_COUNT), 1u);
+            *(uint32_t *)(acc.scratch_of(base + k) + SCR_LIST) = (uint32_t)item;
         }
         return;
     }
