@@ -177,10 +177,10 @@ def test_graph_replays_match_eager(engine_factory, forced):
 @pytest.mark.parametrize("forced", (0, 1), ids=("plain", "forced-retries"))
 def test_two_part_stage2_matches_one_part(engine_factory, restore_config, forced):
     """From 16,384 items on, sign_chunks = 2 runs the linear-algebra pass as
-    two halves on two streams with one failure list: the signatures and
-    attempt counts equal the one-part launch's, also when every item goes
-    through the retry pass, and also replayed from a graph captured across
-    both streams."""
+    two halves on two streams, each with its own failure list and retry
+    pass: the signatures and attempt counts equal the one-part launch's, also
+    when every item goes through the retry pass, and also replayed from a
+    graph captured across both streams."""
     import torch
     v = 1
     eng = restore_config(engine_factory(v))
