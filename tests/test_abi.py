@@ -1,7 +1,9 @@
 """The C ABI: every built library loads and exports exactly the functions
-include/cumayo.h declares, and the Python bindings cover all of them.
-Runs without a GPU (loading the library registers its code objects lazily;
-no entry point is called)."""
+include/cumayo.h declares, and the Python bindings cover all of them; the
+launch configuration calls and the argument checks of the batched entry
+points behave as declared. Runs without a GPU: loading a library registers
+its code objects lazily, and the entry points called here return before
+enqueuing any work."""
 import ctypes
 import os
 import re
