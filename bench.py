@@ -264,6 +264,65 @@ def distinct_keys_run(n: int, steps: int, v: int = 2, mlen: int = 64):
             "all_valid": bool((res.numpy() == 0).all())}
 
 
+def host_cpu() -> dict:
+    """nproc, the cores this process may run on, and the CPU model name
+    (lscpu's "Model name", else /proc/cpuinfo's)."""
+    import subprocess
+    model = ""
+    try:
+        res = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10)
+        for line in res.stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model:
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "available": avail, "model": model}
+
+
+def cpu_baseline(v, d_esk, kidx, flat, offs, lens, sigs, sample, runs=5):
+    """The oracle (C restatement of the reference algorithm, OpenMP over
+    items) signing the first `sample` items of the workload: compute only
+    (the key table is downloaded before the clock starts), median of `runs`
+    runs, on the host cores this process is granted (OMP_NUM_THREADS when
+    the host sets one, else every available core), with the oracle build
+    for the highest x86-64 level the CPU supports."""
+    from oracle import oracle as O
+    cpu = host_cpu()
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(cpu["available"], omp) if omp > 0 else cpu["available"]
+    L, isa = O.baseline_lib()
+    S = min(sample, len(kidx))
+    print(f"[bench] cpu baseline: oracle ({isa}) sign of {S} items on {threads} threads x {runs} runs",
+          file=sys.stderr, flush=True)
+    esk = d_esk.cpu().numpy()
+    times, osigs = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        osigs = O.sign_batch(v, esk, kidx[:S], flat, offs[:S], lens[:S], threads=threads, library=L)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {
+        "value": round(S / med, 1), "unit": "sign/s", "cores": threads, "kind": "port",
+        "sample": f"first {S} signatures of the same workload, median of {runs} runs, compute only; "
+                  f"oracle (C restatement of the reference algorithm, OpenMP over items)",
+        "isa": isa, "nproc": cpu["nproc"], "cores_available": cpu["available"],
+        "omp_num_threads_env": omp or None, "cpu_model": cpu["model"],
+        "run_s": [round(t, 3) for t in times],
+        "bit_exact_vs_gpu": bool(np.array_equal(osigs, sigs[:S])),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -603,21 +662,7 @@ def main():
         out["distinct_keys"] = distinct_keys_run(args.distinct_keys, max(1, args.steps // 2))
 
     if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
-        S = min(args.cpu_sample, n)
-        print(f"[bench] cpu baseline: oracle sign of {S} items on {threads} threads",
-              file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        esk = d_esk.cpu().numpy()
-        osigs = O.sign_batch(v, esk, kidx[:S], flat, offs[:S], lens[:S], threads=threads)
-        cpu_s = time.perf_counter() - t0
-        out["cpu_baseline"] = {
-            "value": round(S / cpu_s, 1), "unit": "sign/s", "cores": threads, "kind": "port",
-            "sample": f"first {S} signatures of the same workload, oracle (C restatement of the "
-                      f"reference algorithm, OpenMP over items)",
-            "bit_exact_vs_gpu": bool(np.array_equal(osigs, sigs[:S])),
-        }
+        out["cpu_baseline"] = cpu_baseline(v, d_esk, kidx, flat, offs, lens, sigs, args.cpu_sample)
     print(json.dumps(out))
 
 

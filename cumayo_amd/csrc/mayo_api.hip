@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "../../include/cumayo.h"
 #include "mayo_kernels.hpp"
@@ -380,17 +381,45 @@ LaunchCfg current_cfg() {
 // part then still holds several rounds of waves per SIMD).
 constexpr size_t CHUNK_MIN = 16384;
 
-// This thread's helper stream on the current device (default priority),
-// created on first use and kept for the life of the thread. Two-part passes
-// issued by one thread on different streams share it, so their second
-// halves run in the order they were issued.
-hipStream_t aux_stream() {
-    thread_local hipStream_t streams[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
-        streams[dev] = nullptr;
-    return streams[dev];
+// Helper streams of two-part passes, per calling thread, keyed by (device,
+// caller stream): passes issued from one thread on different streams (the
+// server's slots, say) get different helpers and stay independent. Each
+// thread keeps at most HELPERS of them, dropping the oldest (a helper is
+// only ever joined through events, so destroying it with work still queued
+// is safe: HIP releases it once that work is done), and destroys the rest
+// when it exits. Helpers run at the default priority.
+class HelperStreams {
+  public:
+    ~HelperStreams() {
+        for (auto &e : v_) (void)hipStreamDestroy(e.helper);
+    }
+    hipStream_t get(hipStream_t caller) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+        for (auto &e : v_)
+            if (e.dev == dev && e.caller == caller) return e.helper;
+        hipStream_t h = nullptr;
+        if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        if (v_.size() >= HELPERS) {
+            (void)hipStreamDestroy(v_.front().helper);
+            v_.erase(v_.begin());
+        }
+        v_.push_back({dev, caller, h});
+        return h;
+    }
+
+  private:
+    static constexpr size_t HELPERS = 16;
+    struct Entry {
+        int dev;
+        hipStream_t caller, helper;
+    };
+    std::vector<Entry> v_;
+};
+
+hipStream_t aux_stream(hipStream_t caller) {
+    thread_local HelperStreams helpers;
+    return helpers.get(caller);
 }
 
 // Stage 2 of signing: attempt 0 in two kernels tuned for their register
@@ -411,7 +440,7 @@ hipError_t launch_sign_stage2(PK prep_k, SK solve_k, const TableSign &acc, size_
         hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3((unsigned)(m < 256 ? m : 256)), dim3(retry_threads), 0, s,
                            acc, n, fail_first, base);
     };
-    hipStream_t aux = parts > 1 ? aux_stream() : nullptr;
+    hipStream_t aux = parts > 1 ? aux_stream(stream) : nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     if (parts > 1 && aux && hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess &&
         hipEventCreateWithFlags(&join, hipEventDisableTiming) == hipSuccess) {
@@ -702,15 +731,15 @@ namespace {
 // instantiated graph; null (and the error noted) on failure.
 template <class F>
 cumayo_graph *capture(F &&launch) {
-    // a two-part signing pass forks onto this thread's helper stream: create
-    // it before capture starts
-    (void)aux_stream();
     hipStream_t s = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e != hipSuccess) {
         note(e);
         return nullptr;
     }
+    // a two-part signing pass forks onto the helper stream of (this thread,
+    // s): create it before capture starts
+    (void)aux_stream(s);
     hipGraph_t g = nullptr;
     e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
@@ -955,6 +984,10 @@ size_t cumayo_serve_poll(void *server, cumayo_completion *out, size_t max) {
 
 void cumayo_serve_get_stats(void *server, cumayo_serve_stats *out) {
     if (server && out) static_cast<serve::Server *>(server)->stats(out);
+}
+
+void cumayo_serve_inject_failures(void *server, int n) {
+    if (server) static_cast<serve::Server *>(server)->inject_failures(n);
 }
 
 int64_t cumayo_serve_clock_ns(void) { return serve::now_ns(); }

@@ -40,6 +40,13 @@ __global__ void k_gather_attempts(const uint8_t *__restrict__ ws, uint8_t *__res
     if (i < n) out[i] = ws[i * mayo::SIGN_SCRATCH_STRIDE + mayo::SCR_ATTEMPTS];
 }
 
+// out[0, n) = v. Verification results start as 1 (invalid), so a result
+// byte that no verification kernel wrote reads as a rejection.
+__global__ void k_fill_bytes(uint8_t *__restrict__ out, uint8_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
 // Fork-join pool for the host-side copies of a batch: run(n, chunk, fn)
 // calls fn(begin, end) over [0, n) in chunks taken by the pool's threads
 // and the caller, and returns when all are done. One job at a time.
@@ -139,6 +146,7 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     bool busy = false;
+    bool failed = false;  // a copy, launch or the batch's event reported an error
     uint8_t *h_msgs = nullptr, *h_sig = nullptr, *h_res = nullptr, *h_att = nullptr;
     int64_t *h_off = nullptr;
     int32_t *h_len = nullptr, *h_key = nullptr;
@@ -291,7 +299,13 @@ class Server {
         out->overlapped_launches = overlapped_;
         out->device_bytes = device_bytes_;
         out->cancelled = cancelled_;
+        out->failed = failed_requests_;
+        out->last_error = err_.load();
     }
+
+    // Test hook: the next n batches fail as if an enqueue had returned an
+    // error (nothing is launched for them).
+    void inject_failures(int n) { inject_.store(n); }
 
     int error() const { return err_.load(); }
     size_t in_system() {
@@ -303,6 +317,10 @@ class Server {
     void ok(hipError_t e) {
         if (e != hipSuccess) err_.store((int)e);
     }
+    void fail(Slot &s, hipError_t e) {
+        s.failed = true;
+        err_.store((int)e);
+    }
     void dev_alloc(uint8_t **p, size_t bytes) {
         ok(hipMalloc((void **)p, bytes ? bytes : 1));
         device_bytes_ += bytes;
@@ -312,11 +330,14 @@ class Server {
         (void)hipSetDevice(cfg_.device);
         for (;;) {
             bool progressed = false;
-            for (auto &s : slots_)
-                if (s.busy && hipEventQuery(s.done) == hipSuccess) {
-                    finish(s);
-                    progressed = true;
-                }
+            for (auto &s : slots_) {
+                if (!s.busy) continue;
+                const hipError_t q = hipEventQuery(s.done);
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) fail(s, q);  // the batch's work ended in an error
+                finish(s);
+                progressed = true;
+            }
             for (auto &s : slots_)
                 if (!s.busy) {
                     if (!launch(s)) break;
@@ -379,30 +400,51 @@ class Server {
             }
         });
         const hipStream_t st = s.stream;
-        ok(hipMemcpyAsync(s.d_msgs, s.h_msgs, (size_t)off + 1, hipMemcpyHostToDevice, st));
-        ok(hipMemcpyAsync(s.d_off, s.h_off, n * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        ok(hipMemcpyAsync(s.d_len, s.h_len, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        ok(hipMemcpyAsync(s.d_key, s.h_key, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        if (s.n_verify)
-            ok(hipMemcpyAsync(s.d_sig + s.n_sign * mayo::SIG_BYTES, s.h_sig + s.n_sign * mayo::SIG_BYTES,
-                              s.n_verify * mayo::SIG_BYTES, hipMemcpyHostToDevice, st));
-        if (s.n_sign) {
-            int e = cumayo_sign_prepared(pesk_, s.d_key, s.d_msgs, s.d_off, s.d_len, s.d_sig, s.n_sign,
-                                         s.d_wss, st, 3);
-            if (e) err_.store(e);
-            ok(hipMemcpyAsync(s.h_sig, s.d_sig, s.n_sign * mayo::SIG_BYTES, hipMemcpyDeviceToHost, st));
-            k_gather_attempts<<<(unsigned)((s.n_sign + 255) / 256), 256, 0, st>>>(s.d_wss, s.d_att, s.n_sign);
-            ok(hipGetLastError());
-            ok(hipMemcpyAsync(s.h_att, s.d_att, s.n_sign, hipMemcpyDeviceToHost, st));
+        // every enqueue is checked: an error fails the whole batch (its
+        // requests complete with STATUS_ERROR and no output is read back)
+        auto chk = [&](int e) {
+            if (e) fail(s, (hipError_t)e);
+        };
+        if (inject_.load() > 0 && inject_.fetch_sub(1) > 0) fail(s, hipErrorLaunchFailure);  // test hook
+        // result staging starts failed-closed: "all attempts failed" for
+        // signing, "invalid" for verification
+        if (s.n_sign) std::memset(s.h_att, 0, s.n_sign);
+        if (s.n_verify) std::memset(s.h_res, 1, s.n_verify);
+        if (!s.failed) chk(hipMemcpyAsync(s.d_msgs, s.h_msgs, (size_t)off + 1, hipMemcpyHostToDevice, st));
+        if (!s.failed) chk(hipMemcpyAsync(s.d_off, s.h_off, n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        if (!s.failed) chk(hipMemcpyAsync(s.d_len, s.h_len, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        if (!s.failed) chk(hipMemcpyAsync(s.d_key, s.h_key, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        if (s.n_verify && !s.failed)
+            chk(hipMemcpyAsync(s.d_sig + s.n_sign * mayo::SIG_BYTES, s.h_sig + s.n_sign * mayo::SIG_BYTES,
+                               s.n_verify * mayo::SIG_BYTES, hipMemcpyHostToDevice, st));
+        if (s.n_sign && !s.failed) {
+            chk(cumayo_sign_prepared(pesk_, s.d_key, s.d_msgs, s.d_off, s.d_len, s.d_sig, s.n_sign, s.d_wss, st,
+                                     3));
+            if (!s.failed)
+                chk(hipMemcpyAsync(s.h_sig, s.d_sig, s.n_sign * mayo::SIG_BYTES, hipMemcpyDeviceToHost, st));
+            if (!s.failed) {
+                k_gather_attempts<<<(unsigned)((s.n_sign + 255) / 256), 256, 0, st>>>(s.d_wss, s.d_att, s.n_sign);
+                chk(hipGetLastError());
+            }
+            if (!s.failed) chk(hipMemcpyAsync(s.h_att, s.d_att, s.n_sign, hipMemcpyDeviceToHost, st));
         }
-        if (s.n_verify) {
-            int e = cumayo_verify_prepared(pepk_, s.d_key + s.n_sign, s.d_msgs, s.d_off + s.n_sign,
+        if (s.n_verify && !s.failed) {
+            k_fill_bytes<<<(unsigned)((s.n_verify + 255) / 256), 256, 0, st>>>(s.d_res, 1, s.n_verify);
+            chk(hipGetLastError());
+            if (!s.failed)
+                chk(cumayo_verify_prepared(pepk_, s.d_key + s.n_sign, s.d_msgs, s.d_off + s.n_sign,
                                            s.d_len + s.n_sign, s.d_sig + s.n_sign * mayo::SIG_BYTES, s.d_res,
-                                           s.n_verify, s.d_wsv, st, 3);
-            if (e) err_.store(e);
-            ok(hipMemcpyAsync(s.h_res, s.d_res, s.n_verify, hipMemcpyDeviceToHost, st));
+                                           s.n_verify, s.d_wsv, st, 3));
+            if (!s.failed) chk(hipMemcpyAsync(s.h_res, s.d_res, s.n_verify, hipMemcpyDeviceToHost, st));
         }
-        ok(hipEventRecord(s.done, st));
+        // the event completes the slot even when nothing was enqueued; if it
+        // cannot be recorded the slot is finished at once, as failed
+        if (hipEventRecord(s.done, st) != hipSuccess) {
+            fail(s, hipErrorInvalidValue);
+            s.busy = true;
+            finish(s);
+            return true;
+        }
         s.busy = true;
         size_t busy = 0;
         for (auto &o : slots_) busy += o.busy ? 1 : 0;
@@ -416,9 +458,15 @@ class Server {
     void finish(Slot &s) {
         const int64_t t = now_ns();
         const size_t n = s.reqs.size();
+        const bool failed = s.failed;
         pool_.run(n, 256, [&](size_t b, size_t e) {
             for (size_t i = b; i < e; ++i) {
                 const cumayo_request &r = s.reqs[i];
+                if (failed) {
+                    // nothing of this batch is trusted: no output is written
+                    s.comps[i] = {r.tag, t - r.t_arrival_ns, CUMAYO_STATUS_ERROR};
+                    continue;
+                }
                 uint8_t status;
                 if (i < s.n_sign) {
                     if (r.sig_out) std::memcpy(r.sig_out, s.h_sig + i * mayo::SIG_BYTES, mayo::SIG_BYTES);
@@ -432,7 +480,9 @@ class Server {
             }
         });
         s.busy = false;
+        s.failed = false;
         std::lock_guard<std::mutex> lk(mu_);
+        if (failed) failed_requests_ += n;
         completed_.insert(completed_.end(), s.comps.begin(), s.comps.begin() + (long)n);
         requests_done_ += s.reqs.size();
         in_flight_requests_ -= s.reqs.size();
@@ -449,8 +499,9 @@ class Server {
     std::thread thread_;
     bool stop_ = false;
     std::atomic<int> err_{0};
+    std::atomic<int> inject_{0};
     uint64_t batches_ = 0, requests_done_ = 0, rejected_ = 0, max_in_flight_ = 0, overlapped_ = 0,
-             cancelled_ = 0;
+             cancelled_ = 0, failed_requests_ = 0;
     size_t in_flight_requests_ = 0, device_bytes_ = 0;
 };
 

@@ -19,6 +19,21 @@ _SZ = ctypes.c_size_t
 OP_SIGN = 0
 OP_VERIFY = 1
 STATUS_REFUSED = 2
+STATUS_CANCELLED = 3
+STATUS_ERROR = 4
+
+
+class ServeError(RuntimeError):
+    """A batch failed on the device: its requests completed with
+    STATUS_ERROR and wrote no output. ``completions`` holds everything the
+    poll returned (the failed requests included), ``hip_error`` the server's
+    last HIP error."""
+
+    def __init__(self, hip_error: int, completions):
+        super().__init__(f"server batch failed: HIP error {hip_error}; "
+                         f"{sum(1 for c in completions if c[1] == STATUS_ERROR)} requests completed with STATUS_ERROR")
+        self.hip_error = hip_error
+        self.completions = completions
 
 
 class ServeConfig(ctypes.Structure):
@@ -43,7 +58,8 @@ class ServeStats(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_uint64), ("requests", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("max_in_flight", ctypes.c_uint64),
                 ("overlapped_launches", ctypes.c_uint64), ("device_bytes", _SZ),
-                ("cancelled", ctypes.c_uint64)]
+                ("cancelled", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("last_error", ctypes.c_int32)]
 
 
 class ServeReport(ctypes.Structure):
@@ -65,6 +81,7 @@ EXPORTS = {
     "cumayo_serve_poll": ([_VP, ctypes.POINTER(Completion), _SZ], _SZ),
     "cumayo_serve_get_stats": ([_VP, ctypes.POINTER(ServeStats)], None),
     "cumayo_serve_clock_ns": ([], ctypes.c_int64),
+    "cumayo_serve_inject_failures": ([_VP, ctypes.c_int], None),
     "cumayo_serve_cancel": ([_VP, ctypes.c_uint64], ctypes.c_int),
     "cumayo_serve_run_poisson": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_uint64, _VP, _VP, _VP, _VP, _VP, _SZ,
@@ -160,7 +177,15 @@ class Server:
         """Withdraws a queued request; it then completes with status 3."""
         return bool(self.lib.cumayo_serve_cancel(self.handle, tag))
 
+    def inject_failures(self, n: int) -> None:
+        """Test hook: the next n batches fail (their requests complete with
+        STATUS_ERROR, nothing is launched for them)."""
+        self.lib.cumayo_serve_inject_failures(self.handle, n)
+
     def poll(self, max_n: int = 65536):
+        """Completed requests as (tag, status, latency_ns, sig, result).
+        Raises ServeError (with every polled completion attached) when one of
+        them belongs to a batch that failed on the device."""
         buf = (Completion * max_n)()
         k = int(self.lib.cumayo_serve_poll(self.handle, buf, max_n))
         out = []
@@ -168,10 +193,13 @@ class Server:
             c = buf[i]
             m, out_sig, out_res, _ = self._live.pop(c.tag)
             out.append((c.tag, c.status, c.latency_ns, out_sig, int(out_res[0])))
+        if any(c[1] == STATUS_ERROR for c in out):
+            raise ServeError(self.stats()["last_error"], out)
         return out
 
     def wait(self, tags, timeout_s: float = 60.0):
-        """Polls until every tag has completed; returns {tag: (status, latency_ns, sig, result)}."""
+        """Polls until every tag has completed; returns {tag: (status, latency_ns, sig, result)}.
+        A failed batch raises ServeError from poll()."""
         want = set(tags)
         got = {}
         t_end = time.monotonic() + timeout_s

@@ -258,13 +258,20 @@ typedef struct {
     uint8_t status;         /* verify: 0 valid / 1 invalid; sign: 0 ok, 1 all attempts failed;
                                2 = refused (key out of range, no key table for the op,
                                or a message longer than slot_msg_bytes);
-                               3 = cancelled before it was launched */
+                               3 = cancelled before it was launched;
+                               4 = CUMAYO_STATUS_ERROR: a copy or launch of the request's
+                               batch failed, or its work ended in a HIP error; no output
+                               (signature, result byte) was written */
 } cumayo_completion;
+
+#define CUMAYO_STATUS_ERROR 4
 
 typedef struct {
     uint64_t batches, requests, rejected, max_in_flight, overlapped_launches;
     size_t device_bytes;
     uint64_t cancelled;
+    uint64_t failed;        /* requests completed with CUMAYO_STATUS_ERROR */
+    int32_t last_error;     /* the last HIP error the server met (0 = none) */
 } cumayo_serve_stats;
 
 typedef struct {
@@ -284,6 +291,10 @@ size_t cumayo_serve_poll(void *server, cumayo_completion *out, size_t max);
  * if it had already been launched or completed. */
 int cumayo_serve_cancel(void *server, uint64_t tag);
 void cumayo_serve_get_stats(void *server, cumayo_serve_stats *out);
+/* Test hook: the next n batches the server launches fail as if an enqueue
+ * had returned an error; nothing is launched for them and their requests
+ * complete with CUMAYO_STATUS_ERROR. */
+void cumayo_serve_inject_failures(void *server, int n);
 int64_t cumayo_serve_clock_ns(void);
 /* Open-loop load: Poisson arrivals at rate_per_s for duration_s over a pool
  * of prepared requests (message i of the pool, key msg_key[i], and for
@@ -320,9 +331,9 @@ typedef struct {
     int32_t verify_order;
     int32_t sign_chunks;  /* 1, or 2: from 16,384 items on, the signing linear-algebra pass
                              runs as two halves on two streams (the second on a helper
-                             stream of the calling thread, forked from and joined back to
-                             the caller's stream), so that each half fills the other's
-                             kernel tails */
+                             stream kept per calling thread and caller stream, forked from
+                             and joined back to the caller's stream), so that each half
+                             fills the other's kernel tails */
 } cumayo_launch_config;
 
 #define CUMAYO_FIELD_SIGN_PREP 0

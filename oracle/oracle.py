@@ -1,8 +1,11 @@
 """ctypes front end of the CPU oracle (TEST INFRASTRUCTURE).
 
-Loads ``oracle/libmayo_oracle.so`` (built by ``oracle/Makefile``). Only the
-tests, the smoke check and the CPU-baseline leg of ``bench.py`` import this
-module; the product path in ``cumayo_amd`` never does.
+Loads ``oracle/libmayo_oracle.so`` (built by ``oracle/Makefile`` for the
+portable x86-64-v2 level). ``baseline_lib()`` loads the copy built for the
+highest x86-64 level the running CPU supports (v4 = AVX-512, v3 = AVX2),
+which the CPU-baseline leg times. Only the tests, the smoke check and the
+CPU-baseline leg of ``bench.py`` import this module; the product path in
+``cumayo_amd`` never does.
 """
 from __future__ import annotations
 
@@ -15,6 +18,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libmayo_oracle.so")
 _lib = None
+_baseline = None
+# x86-64 micro-architecture levels and the CPU flags each needs beyond the
+# previous one (the levels of the psABI supplement)
+_ISA_FLAGS = {
+    "x86-64-v3": ("avx", "avx2", "bmi1", "bmi2", "f16c", "fma", "abm", "movbe", "xsave"),
+    "x86-64-v4": ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"),
+}
 
 VERSIONS = (1, 2, 3, 5)
 
@@ -57,46 +67,82 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
-        L = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.POINTER
-        u8p = P(ctypes.c_uint8)
-        L.mayo_oracle_params.restype = P(_Params)
-        L.mayo_oracle_params.argtypes = [ctypes.c_int]
-        for fn in ("oracle_shake128", "oracle_shake256"):
-            getattr(L, fn).argtypes = [u8p, ctypes.c_size_t, u8p, ctypes.c_size_t]
-        L.oracle_keccak_f1600.argtypes = [P(ctypes.c_uint64)]
-        L.oracle_aes128_ecb_block.argtypes = [u8p, u8p, u8p]
-        L.oracle_aes128_ctr.argtypes = [u8p, u8p, ctypes.c_size_t]
-        L.oracle_gf16_mul.restype = ctypes.c_uint8
-        L.oracle_gf16_mul.argtypes = [ctypes.c_uint8, ctypes.c_uint8]
-        L.oracle_gf16_inv.restype = ctypes.c_uint8
-        L.oracle_gf16_inv.argtypes = [ctypes.c_uint8]
-        L.oracle_mvec_E.argtypes = [ctypes.c_int, u8p]
-        L.oracle_keypair_seed.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, u8p]
-        L.oracle_keypair.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, u8p, u8p]
-        L.oracle_keypair_from_sk.argtypes = [ctypes.c_int, u8p, u8p]
-        L.oracle_expandsk.argtypes = [ctypes.c_int, u8p, u8p]
-        L.oracle_expandpk.argtypes = [ctypes.c_int, u8p, u8p]
-        L.oracle_sign.restype = ctypes.c_int
-        L.oracle_sign.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_size_t, u8p, ctypes.c_int]
-        L.oracle_verify.restype = ctypes.c_int
-        L.oracle_verify.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_size_t, u8p]
-        L.oracle_sign_digest_salt_t.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_size_t, u8p, u8p, u8p]
-        L.oracle_sign_vr.argtypes = [ctypes.c_int, u8p, u8p, u8p, ctypes.c_int, u8p]
-        L.oracle_sign_M.argtypes = [ctypes.c_int, u8p, u8p, u8p]
-        L.oracle_sign_A.argtypes = [ctypes.c_int, u8p, u8p]
-        L.oracle_sign_U.argtypes = [ctypes.c_int, u8p, u8p, u8p]
-        L.oracle_sign_y.argtypes = [ctypes.c_int, u8p, u8p, u8p]
-        L.oracle_sample_solution.restype = ctypes.c_int
-        L.oracle_sample_solution.argtypes = [ctypes.c_int, ctypes.c_int, u8p, u8p, u8p]
-        L.oracle_eval_public.argtypes = [ctypes.c_int, u8p, u8p, u8p]
-        i32p, i64p = P(ctypes.c_int32), P(ctypes.c_int64)
-        L.oracle_sign_batch.argtypes = [ctypes.c_int, u8p, i32p, u8p, i64p, i32p, u8p,
-                                        ctypes.c_int64, ctypes.c_int]
-        L.oracle_verify_batch.argtypes = [ctypes.c_int, u8p, i32p, u8p, i64p, i32p, u8p, u8p,
-                                          ctypes.c_int64, ctypes.c_int]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def cpu_isa_level() -> str:
+    """The highest x86-64 level (v2, v3, v4) whose flags /proc/cpuinfo lists."""
+    flags = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        pass
+    level = "x86-64-v2"
+    for name in ("x86-64-v3", "x86-64-v4"):
+        if not all(fl in flags for fl in _ISA_FLAGS[name]):
+            break
+        level = name
+    return level
+
+
+def baseline_lib():
+    """(library, isa): the oracle built for the highest x86-64 level this CPU
+    supports (falling back to the portable build)."""
+    global _baseline
+    if _baseline is None:
+        isa = cpu_isa_level()
+        path = os.path.join(_HERE, f"libmayo_oracle_{isa}.so")
+        if isa == "x86-64-v2" or not os.path.exists(path):
+            _baseline = (lib(), "x86-64-v2")
+        else:
+            _baseline = (_bind(ctypes.CDLL(path)), isa)
+    return _baseline
+
+
+def _bind(L):
+    P = ctypes.POINTER
+    u8p = P(ctypes.c_uint8)
+    L.mayo_oracle_params.restype = P(_Params)
+    L.mayo_oracle_params.argtypes = [ctypes.c_int]
+    for fn in ("oracle_shake128", "oracle_shake256"):
+        getattr(L, fn).argtypes = [u8p, ctypes.c_size_t, u8p, ctypes.c_size_t]
+    L.oracle_keccak_f1600.argtypes = [P(ctypes.c_uint64)]
+    L.oracle_aes128_ecb_block.argtypes = [u8p, u8p, u8p]
+    L.oracle_aes128_ctr.argtypes = [u8p, u8p, ctypes.c_size_t]
+    L.oracle_gf16_mul.restype = ctypes.c_uint8
+    L.oracle_gf16_mul.argtypes = [ctypes.c_uint8, ctypes.c_uint8]
+    L.oracle_gf16_inv.restype = ctypes.c_uint8
+    L.oracle_gf16_inv.argtypes = [ctypes.c_uint8]
+    L.oracle_mvec_E.argtypes = [ctypes.c_int, u8p]
+    L.oracle_keypair_seed.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, u8p]
+    L.oracle_keypair.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, u8p, u8p]
+    L.oracle_keypair_from_sk.argtypes = [ctypes.c_int, u8p, u8p]
+    L.oracle_expandsk.argtypes = [ctypes.c_int, u8p, u8p]
+    L.oracle_expandpk.argtypes = [ctypes.c_int, u8p, u8p]
+    L.oracle_sign.restype = ctypes.c_int
+    L.oracle_sign.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_size_t, u8p, ctypes.c_int]
+    L.oracle_verify.restype = ctypes.c_int
+    L.oracle_verify.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_size_t, u8p]
+    L.oracle_sign_digest_salt_t.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_size_t, u8p, u8p, u8p]
+    L.oracle_sign_vr.argtypes = [ctypes.c_int, u8p, u8p, u8p, ctypes.c_int, u8p]
+    L.oracle_sign_M.argtypes = [ctypes.c_int, u8p, u8p, u8p]
+    L.oracle_sign_A.argtypes = [ctypes.c_int, u8p, u8p]
+    L.oracle_sign_U.argtypes = [ctypes.c_int, u8p, u8p, u8p]
+    L.oracle_sign_y.argtypes = [ctypes.c_int, u8p, u8p, u8p]
+    L.oracle_sample_solution.restype = ctypes.c_int
+    L.oracle_sample_solution.argtypes = [ctypes.c_int, ctypes.c_int, u8p, u8p, u8p]
+    L.oracle_eval_public.argtypes = [ctypes.c_int, u8p, u8p, u8p]
+    i32p, i64p = P(ctypes.c_int32), P(ctypes.c_int64)
+    L.oracle_sign_batch.argtypes = [ctypes.c_int, u8p, i32p, u8p, i64p, i32p, u8p,
+                                    ctypes.c_int64, ctypes.c_int]
+    L.oracle_verify_batch.argtypes = [ctypes.c_int, u8p, i32p, u8p, i64p, i32p, u8p, u8p,
+                                      ctypes.c_int64, ctypes.c_int]
+    return L
 
 
 def params(version: int) -> Params:
@@ -299,7 +345,7 @@ def _i64(a):
 
 
 def sign_batch(version: int, esk_table: np.ndarray, key_idx: np.ndarray, msgs: np.ndarray,
-               msg_off: np.ndarray, msg_len: np.ndarray, threads: int = 0) -> np.ndarray:
+               msg_off: np.ndarray, msg_len: np.ndarray, threads: int = 0, library=None) -> np.ndarray:
     p = params(version)
     n = int(key_idx.size)
     sigs = np.zeros((n, p.sig_bytes), np.uint8)
@@ -308,7 +354,8 @@ def sign_batch(version: int, esk_table: np.ndarray, key_idx: np.ndarray, msgs: n
     mm = np.ascontiguousarray(msgs, dtype=np.uint8)
     mo = np.ascontiguousarray(msg_off, dtype=np.int64)
     ml = np.ascontiguousarray(msg_len, dtype=np.int32)
-    lib().oracle_sign_batch(version, _u8(et), _i32(ki), _u8(mm), _i64(mo), _i32(ml), _u8(sigs), n, threads)
+    (library or lib()).oracle_sign_batch(version, _u8(et), _i32(ki), _u8(mm), _i64(mo), _i32(ml), _u8(sigs), n,
+                                         threads)
     return sigs
 
 

@@ -96,3 +96,60 @@ def test_long_messages_resident_keys(engine_factory, v):
                          threads=THREADS)
     bad = np.where((g != osigs).any(axis=1))[0]
     assert bad.size == 0, f"mismatching items {sample[bad[:8]].tolist()}"
+
+
+def test_headline_batch_through_captured_graphs(engine_factory):
+    """The headline workload at full size: MAYO_1, 65,536 signatures under
+    64 resident keys (key j mod 64, prepared tables), message lengths
+    uniform in [32, 1024] bytes (the bench's generator), signed and verified
+    through the captured sign and verify graphs, replayed twice with the
+    outputs cleared in between: both replays give the eager bytes, every
+    signature verifies, a flipped signature byte is rejected for a strided
+    sample, and all 65,536 signatures equal the oracle's."""
+    import torch
+    import bench
+    v, n, nkeys = 1, 65536, 64
+    eng = engine_factory(v)
+    z = eng.sizes
+    print(f"[config] MAYO_1 headline batch through graphs n={n}", flush=True)
+    d_esk, d_epk = eng.keypair_tables(nkeys, rand_index=0)
+    p_esk, p_epk = eng.prepare_tables(d_esk, d_epk)
+    flat, offs, lens, kidx = bench.make_workload(n, nkeys, 32, 1024)
+    dev = eng.device
+    t = lambda a: torch.from_numpy(np.array(a)).to(dev)
+    d_kidx, d_msgs, d_off, d_len = t(kidx), t(flat), t(offs), t(lens)
+    ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+    eng.sign_prepared(p_esk, d_kidx, d_msgs, d_off, d_len, ref, eng.sign_workspace(n))
+    torch.cuda.synchronize()
+    sigs = torch.zeros_like(ref)
+    res = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    ws_s, ws_v = eng.sign_workspace(n), eng.verify_workspace(n)
+    gs = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, sigs, ws_s)
+    gv = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, sigs, res, ws_v)
+    try:
+        for rep in range(2):
+            sigs.zero_()
+            res.fill_(7)
+            torch.cuda.synchronize()
+            gs.launch()
+            gv.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(sigs, ref), f"sign graph replay {rep}"
+            assert int((res != 0).sum().item()) == 0, f"verify graph replay {rep}"
+        # rejections through the verification graph
+        idx = torch.arange(0, n, 257, device=dev)
+        sigs[idx, 100] ^= 0x01
+        gv.launch()
+        torch.cuda.synchronize()
+        r = res.cpu().numpy()
+        assert r[idx.cpu().numpy()].tolist() == [1] * idx.numel()
+        assert int(r.sum()) == idx.numel()
+    finally:
+        gs.close()
+        gv.close()
+    att = eng.sign_attempts(ws_s, n)
+    assert (att >= 1).all()
+    osigs = O.sign_batch(v, d_esk.cpu().numpy(), kidx, flat, offs, lens, threads=THREADS)
+    g = ref.cpu().numpy()
+    bad = np.where((g != osigs).any(axis=1))[0]
+    assert bad.size == 0, f"mismatching items {bad[:8].tolist()}"

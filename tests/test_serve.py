@@ -277,3 +277,143 @@ def test_serve_key_table_beyond_one_preparation_call(engine_factory):
         st, _, sig, _ = got[i]
         assert st == 0 and sig.tobytes() == batch_sigs[i].tobytes(), (i, int(keys[i]))
         assert got[n + i][0] == int(batch_res[i]), (i, int(keys[i]))
+
+
+def test_serve_failed_batch_fails_closed(engine_factory):
+    """A batch whose enqueue fails (forced from the host by the server's
+    test hook; no kernel runs for it) completes every request with
+    STATUS_ERROR and writes no output: a verification of a valid signature
+    does not come back valid. poll() raises ServeError carrying those
+    completions, the server reports the error in its stats, and the next
+    batch is served normally."""
+    from cumayo_amd.serve import STATUS_ERROR, Server, ServeError
+    eng = engine_factory(1)
+    esk, epk = eng.keypair_tables(2, rand_index=21)
+    reqs, batch_sigs, expect = _requests(eng, esk, epk, 40, 2, seed=5)
+    good_verifies = [r for r in reqs if r[1] == 1 and expect[r[0]] == 0][:4]
+    signs = [r for r in reqs if r[1] == 0][:4]
+    assert good_verifies and signs
+    first = good_verifies + signs
+    with Server(eng, esk, epk, max_batch=64, n_slots=1) as srv:
+        srv.inject_failures(1)
+        assert srv.submit(first) == len(first)
+        got = []
+        t_end = time.monotonic() + 60
+        while len(got) < len(first):
+            try:
+                got += srv.poll()
+            except ServeError as e:
+                got += e.completions
+                assert e.hip_error != 0
+            assert time.monotonic() < t_end, "failed batch never completed"
+            time.sleep(0.001)
+        for tag, status, _, sig, res in got:
+            assert status == STATUS_ERROR, f"tag {tag} status {status}"
+            assert res == 0xEE and not sig.any(), f"tag {tag}: output written for a failed batch"
+        st = srv.stats()
+        assert st["failed"] == len(first) and st["last_error"] != 0
+        # the server keeps serving: the same requests again, now bit-exact
+        again = [(tag + 1000, op, key, msg, sig) for tag, op, key, msg, sig in first]
+        res2 = srv.wait([r[0] for r in again]) if srv.submit(again) == len(again) else {}
+    assert len(res2) == len(again)
+    for tag, op, *_ in again:
+        status, _, sig, res = res2[tag]
+        if op == 1:
+            assert status == 0 and res == 0
+        else:
+            assert status == 0 and sig.tobytes() == batch_sigs[tag - 1000].tobytes()
+
+
+def test_serve_four_set_poisson_mix(engine_factory):
+    """The serve configuration's shape: MAYO_1/2/3/5 requests in a 40/20/20/20
+    mix, half of them verifications (a quarter of those of corrupted
+    signatures or messages), message lengths log-uniform in [32 B, 16 KB],
+    64 resident keys per set, submitted at Poisson arrival times into one
+    MixedServer (one server per set, all on the one GPU). Every signature
+    equals its set's batch signature, every verification its batch verdict,
+    and a strided sample of each set's signatures equals the oracle."""
+    import torch
+    import bench
+    from cumayo_amd.mixed import MixedServer
+    weights = {1: 0.4, 2: 0.2, 3: 0.2, 5: 0.2}
+    total, nkeys = 1000, 64
+    rng = np.random.default_rng(55)
+    engines, tables, data = {}, {}, {}
+    for v, w in weights.items():
+        eng = engine_factory(v)
+        z = eng.sizes
+        n = int(total * w)
+        esk, epk = eng.keypair_tables(nkeys, rand_index=0)
+        flat, offs, lens, kidx = bench.log_uniform_workload(n, nkeys, 32, 16384, tag=f"serve-mix-{v}")
+        dev = eng.device
+        t = lambda a: torch.from_numpy(np.array(a)).to(dev)
+        sigs = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=dev)
+        eng.sign_table(esk, t(kidx), t(flat), t(offs), t(lens), sigs, eng.sign_workspace(n))
+        torch.cuda.synchronize()
+        batch_sigs = sigs.cpu().numpy()
+        ops = (rng.random(n) < 0.5).astype(np.uint8)
+        vmsgs, vsigs = [], []
+        for i in range(n):
+            msg = bytearray(flat[offs[i]:offs[i] + lens[i]].tobytes())
+            sig = bytearray(batch_sigs[i].tobytes())
+            if ops[i] and rng.random() < 0.25:
+                if rng.random() < 0.5:
+                    msg[int(rng.integers(0, len(msg)))] ^= 0x20
+                else:
+                    sig[int(rng.integers(0, z.sig_bytes))] ^= 0x02
+            vmsgs.append(bytes(msg))
+            vsigs.append(bytes(sig))
+        # the batch verdicts of exactly the (message, signature) pairs served
+        vl = np.array([len(m) for m in vmsgs], np.int32)
+        vo = np.concatenate([[0], np.cumsum(vl)[:-1]]).astype(np.int64)
+        vflat = np.frombuffer(b"".join(vmsgs), np.uint8)
+        vres = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+        eng.verify_table(epk, t(kidx), t(vflat), t(vo), t(vl),
+                         t(np.frombuffer(b"".join(vsigs), np.uint8).reshape(n, z.sig_bytes)), vres,
+                         eng.verify_workspace(n))
+        torch.cuda.synchronize()
+        engines[v], tables[v] = eng, (esk, epk)
+        data[v] = dict(n=n, ops=ops, kidx=kidx, vmsgs=vmsgs, vsigs=vsigs, flat=flat, offs=offs,
+                       lens=lens, batch_sigs=batch_sigs, vres=vres.cpu().numpy(), esk=esk)
+    # one arrival stream over the mix: set by weight, exponential gaps at 4,000/s
+    reqs = []
+    nxt = {v: 0 for v in weights}
+    order = [v for v in weights for _ in range(data[v]["n"])]
+    rng.shuffle(order)
+    arrivals = np.cumsum(rng.exponential(1 / 4000.0, len(order)))
+    for k, v in enumerate(order):
+        i = nxt[v]
+        nxt[v] += 1
+        d = data[v]
+        op = int(d["ops"][i])
+        tag = v * 100000 + i
+        reqs.append((float(arrivals[k]), (tag, v, op, int(d["kidx"][i]), d["vmsgs"][i],
+                                                 d["vsigs"][i] if op else None)))
+    print(f"[serve] four-set mix, {len(reqs)} Poisson arrivals", flush=True)
+    with MixedServer(engines, tables, max_batch=64, n_slots=3, slot_msg_bytes=64 * 16384) as ms:
+        t0 = time.perf_counter()
+        k = 0
+        while k < len(reqs):
+            now = time.perf_counter() - t0
+            due = []
+            while k < len(reqs) and reqs[k][0] <= now:
+                due.append(reqs[k][1])
+                k += 1
+            if due:
+                assert ms.submit(due) == len(due)
+            else:
+                time.sleep(min(0.0005, max(0.0, reqs[k][0] - now)))
+        got = ms.wait({v: [v * 100000 + i for i in range(data[v]["n"])] for v in weights})
+    for v, d in data.items():
+        for i in range(d["n"]):
+            gv, status, _, sig, res = got[v * 100000 + i]
+            assert gv == v
+            if d["ops"][i]:
+                assert status == d["vres"][i] and res == d["vres"][i], f"MAYO_{v} verify {i}"
+            else:
+                assert status == 0 and sig.tobytes() == d["batch_sigs"][i].tobytes(), f"MAYO_{v} sign {i}"
+        signs = np.flatnonzero(d["ops"] == 0)[::7]
+        osigs = O.sign_batch(v, d["esk"].cpu().numpy(), d["kidx"][signs], d["flat"], d["offs"][signs],
+                             d["lens"][signs], threads=16)
+        assert np.array_equal(osigs, d["batch_sigs"][signs]), f"MAYO_{v} oracle sample"
+        assert d["vres"][d["ops"] == 1].sum() > 0 and (d["vres"][d["ops"] == 1] == 0).sum() > 0
