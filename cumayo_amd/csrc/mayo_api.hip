@@ -627,6 +627,60 @@ int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const 
     return sign_table_impl<wv::PrepKeys>(acc, n, stream, stages, current_cfg());
 }
 
+// cumayo_sign_prepared with its workspace in `scratch`, a file-scope device
+// array of CUMAYO_NOALLOC_MAX_SIGN items' workspace slots: no allocation,
+// for callers that cannot allocate. Every GPU holds its own copy of the
+// array (until the process ends) at an address that hipGetSymbolAddress
+// returns with that GPU current; the name `scratch` in host code is only a
+// host-side stand-in, so host code uses scratch_ptr() and never the name.
+// Nothing in the workspace needs clearing between calls (the kernels reset
+// the failure-list count themselves and write every slot before reading
+// it); calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) uint8_t scratch[(size_t)CUMAYO_NOALLOC_MAX_SIGN * SIGN_SCRATCH_STRIDE];
+
+namespace {
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[64] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per GPU;
+// null (error noted) if the lookup fails.
+uint8_t *scratch_ptr() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64) {
+        note(e != hipSuccess ? e : hipErrorInvalidDevice);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            note(e != hipSuccess ? e : hipErrorInvalidSymbol);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<uint8_t *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int cumayo_sign_prepared_noalloc(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                                 const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                                 hipStream_t stream) {
+    if (!n) return 0;
+    if (n > CUMAYO_NOALLOC_MAX_SIGN) {
+        note(hipErrorInvalidValue);
+        return (int)hipErrorInvalidValue;
+    }
+    uint8_t *ws = scratch_ptr();
+    if (!ws) return (int)hipErrorInvalidSymbol;
+    // one launch part (n is far below the two-part threshold): no helper
+    // stream or event is created
+    static_assert(CUMAYO_NOALLOC_MAX_SIGN < CHUNK_MIN, "the no-allocation path runs stage 2 in one part");
+    return cumayo_sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, ws, stream, 3);
+}
+
 int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
                            const int64_t *msg_off, const int32_t *msg_len, const uint8_t *sigs,
                            uint8_t *results, size_t n, uint8_t *workspace, hipStream_t stream,

@@ -125,6 +125,7 @@ EXPORTS = {
     "cumayo_prepare_epk": [_VP, _SZ, _SZ, _VP, _VP],
     "cumayo_sign_prepared": [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
     "cumayo_verify_prepared": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP, ctypes.c_int],
+    "cumayo_sign_prepared_noalloc": [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP],
     "cumayo_crypto_sign_workspace_size": [_SZ],
     "cumayo_crypto_sign": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
     "cumayo_crypto_sign_open_workspace_size": [_SZ],
@@ -153,6 +154,7 @@ _RESTYPES = {
     "cumayo_prepare_epk": ctypes.c_int,
     "cumayo_sign_prepared": ctypes.c_int,
     "cumayo_verify_prepared": ctypes.c_int,
+    "cumayo_sign_prepared_noalloc": ctypes.c_int,
     "cumayo_crypto_sign_workspace_size": _SZ,
     "cumayo_crypto_sign": ctypes.c_int,
     "cumayo_crypto_sign_open_workspace_size": _SZ,

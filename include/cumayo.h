@@ -191,6 +191,23 @@ int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, cons
                            uint8_t *results, size_t n, uint8_t *workspace, hipStream_t stream,
                            int stages);
 
+/* cumayo_sign_prepared (stages 3) without a workspace argument and without
+ * allocating: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The workspace is a file-scope device
+ * array of CUMAYO_NOALLOC_MAX_SIGN items' workspace slots that each GPU holds
+ * for the life of the process (CUMAYO_NOALLOC_MAX_SIGN x sign_ws_stride
+ * bytes: 7.5 MiB for MAYO_1, 23 MiB for MAYO_5); its address is looked up once
+ * per GPU (hipGetSymbolAddress) on the first call with that GPU current.
+ * Batches above CUMAYO_NOALLOC_MAX_SIGN items return hipErrorInvalidValue
+ * with nothing enqueued. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race and must be
+ * serialised by the caller. Same results as cumayo_sign_prepared. */
+#define CUMAYO_NOALLOC_MAX_SIGN 2048u
+int cumayo_sign_prepared_noalloc(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                                 const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                                 hipStream_t stream);
+
 /* NIST wire format (the crypto_sign / crypto_sign_open convention of the
  * MAYO submission: sm = signature || message), batched over prepared key
  * tables; no counterpart in the reference, which exposes detached signatures
