@@ -93,6 +93,21 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | (uint64_t)uni((uint32_t)x);
 }
 
+// Scalar-unit ops on wave-uniform values. Written out so that the compiler
+// keeps them on the SALU: left to itself it fuses and-not / xor chains of
+// uniform values into v_bitop3 on the vector unit, which is the busier port
+// of the elimination.
+__device__ __forceinline__ uint32_t s_xor(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("s_xor_b32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t s_andn2(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("s_andn2_b32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+    return r;
+}
+
 // Barrier policies of the helpers below: the whole workgroup, or only the
 // one wave that runs them while the others wait at the caller's next
 // barrier. A wave's LDS accesses complete in program order, so one wave
@@ -104,6 +119,18 @@ struct BlockSync {
 struct WaveSync {
     __device__ static void sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 };
+
+// XOR of x over the 64 lanes of the wave (uniform result; every lane of the
+// wave must be active): within each row of 16 lanes by DPP (quad swaps, then
+// rotations by 4 and 8), then across the four rows on the scalar unit.
+__device__ __forceinline__ uint32_t wave_xor(uint32_t x) {
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false);  // row_ror:4
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    return __builtin_amdgcn_readlane(x, 0) ^ __builtin_amdgcn_readlane(x, 16) ^
+           __builtin_amdgcn_readlane(x, 32) ^ __builtin_amdgcn_readlane(x, 48);
+}
 
 // Multiply a bitsliced nibble vector (planes p0..p3) by a.
 __device__ __forceinline__ void bs_mul_a(uint32_t &p0, uint32_t &p1, uint32_t &p2, uint32_t &p3) {
@@ -130,7 +157,6 @@ struct Shared {
     uint32_t yu[YG_MAX * UW];    // unreduced y (partial sums of the pair groups)
     uint32_t y[MW];              // y = t + sum E^l u_l
     uint32_t t[MW];
-    uint32_t Ar[4 * NG];         // A r, bitsliced
     uint32_t yplanes[4 * NG];    // final y' column, bitsliced
     int16_t prow[KO];            // pivot row of each column, -1 if free
     uint8_t pinv[KO];            // inverse of each column's pivot value
@@ -607,7 +633,6 @@ struct SolveShared {
     uint8_t vr[VR_BYTES];
     uint32_t Mb[KO * 4 * NG];
     uint32_t y[MW];
-    uint32_t Ar[4 * NG];
     uint32_t yplanes[4 * NG];
     int16_t prow[KO];
     uint8_t pinv[KO];
@@ -771,6 +796,15 @@ __device__ __forceinline__ void dispatch_word(int g, F &&f) {
     }
 }
 
+// f(integral_constant<int, i>) for i = I .. N-1, in order
+template <int N, int I = 0, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
 // Gauss-Jordan on the columns of [A | y - A r]; x (S.x) = r + z. Returns 1
 // on full rank. Each pivot step: the pivot column f is broadcast (SGPRs),
 // its pivot row pr is the lowest unused row with a nonzero entry, p = f[pr];
@@ -784,9 +818,9 @@ template <class Sync = BlockSync, class LDS>
 __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = false) {
     const int lane = threadIdx.x;
     const uint8_t *rv = S.vr + R_OFF;
-    // y' = y - A r: every A column times its r nibble, XOR-reduced over lanes
-    if (lane < 4 * NG) S.Ar[lane] = 0;
-    Sync::sync();
+    // y' = y - A r: every A column times its r nibble, XOR-reduced over the
+    // lanes by DPP (an LDS atomic at one address per word would become a
+    // loop over the active lanes)
     {
         uint32_t part[4][NG];
 #pragma unroll
@@ -797,13 +831,18 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
         for (int s = 0; s < NCL; ++s) {
             const int c = col_of(s, lane);
             const uint32_t rc = c < KO ? mvec::nib(rv, c) : 0u;
+            uint32_t rm[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) rm[b] = gf16::bitmask(rc, b);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
                 uint32_t p0 = cols[s][0][g], p1 = cols[s][1][g], p2 = cols[s][2][g], p3 = cols[s][3][g];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const uint32_t m = gf16::bitmask(rc, b);
-                    part[0][g] ^= p0 & m; part[1][g] ^= p1 & m; part[2][g] ^= p2 & m; part[3][g] ^= p3 & m;
+                    part[0][g] = gf16::axm(part[0][g], p0, rm[b]);
+                    part[1][g] = gf16::axm(part[1][g], p1, rm[b]);
+                    part[2][g] = gf16::axm(part[2][g], p2, rm[b]);
+                    part[3][g] = gf16::axm(part[3][g], p3, rm[b]);
                     bs_mul_a(p0, p1, p2, p3);
                 }
             }
@@ -811,25 +850,21 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll
-            for (int g = 0; g < NG; ++g)
-                if (part[d][g]) atomicXor(&S.Ar[d * NG + g], part[d][g]);
+            for (int g = 0; g < NG; ++g) {
+                const uint32_t ar = wave_xor(part[d][g]);
+                if (lane == Y_LANE) cols[Y_SET][d][g] ^= ar;
+            }
     }
-    Sync::sync();
-    if (lane == Y_LANE) {
+    constexpr int NH = (NG + 1) / 2;
+    uint64_t used[NH];
 #pragma unroll
-        for (int d = 0; d < 4; ++d)
-#pragma unroll
-            for (int g = 0; g < NG; ++g) cols[Y_SET][d][g] ^= S.Ar[d * NG + g];
-    }
-    uint32_t used[NG];
-#pragma unroll
-    for (int g = 0; g < NG; ++g) used[g] = 0;
+    for (int h = 0; h < NH; ++h) used[h] = 0;
     int rank = 0;
     // pivots in column order: set NCL-1 (columns < SHIFT) first, then 0, 1, ...
-#pragma unroll
-    for (int ord = 0; ord < NCL; ++ord) {
-        const int ps = ord == 0 ? NCL - 1 : ord - 1;  // set holding these pivot columns
-        const int nl = ps == NCL - 1 ? SHIFT : 64;
+    static_for<NCL>([&](auto Oc) {
+        constexpr int ord = decltype(Oc)::value;
+        constexpr int ps = ord == 0 ? NCL - 1 : ord - 1;  // set holding these pivot columns
+        constexpr int nl = ps == NCL - 1 ? SHIFT : 64;
 #pragma unroll 1
         for (int pl = 0; pl < nl; ++pl) {
             const int pc = col_of(ps, pl);
@@ -838,74 +873,117 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
             // unused row with a nonzero entry (a scalar search, which
             // measured faster here than every lane searching its own column
             // on the vector unit)
-            uint32_t f[4][NG];
+            // (64-bit pairs of words: the scalar unit has 64-bit logic ops,
+            // and uniform 64-bit values stay on it)
+            uint64_t fw[4][NH];
 #pragma unroll
             for (int d = 0; d < 4; ++d)
 #pragma unroll
-                for (int g = 0; g < NG; ++g) f[d][g] = __builtin_amdgcn_readlane(cols[ps][d][g], pl);
+                for (int h = 0; h < NH; ++h) {
+                    const uint32_t lo = __builtin_amdgcn_readlane(cols[ps][d][2 * h], pl);
+                    const uint32_t hi = 2 * h + 1 < NG ? __builtin_amdgcn_readlane(cols[ps][d][2 * h + 1], pl) : 0u;
+                    fw[d][h] = ((uint64_t)hi << 32) | lo;
+                }
             int pr = -1;
 #pragma unroll
-            for (int g = NG - 1; g >= 0; --g) {
-                const uint32_t nz = (f[0][g] | f[1][g] | f[2][g] | f[3][g]) & ~used[g];
-                pr = nz ? 32 * g + (int)__builtin_ctz(nz) : pr;
+            for (int h = NH - 1; h >= 0; --h) {
+                const uint64_t nz = (fw[0][h] | fw[1][h] | fw[2][h] | fw[3][h]) & ~used[h];
+                pr = nz ? 64 * h + (int)__builtin_ctzll(nz) : pr;
             }
             pr = __builtin_amdgcn_readfirstlane(pr);
             if (pr >= 0) {
-                const int pb = pr & 31;
-                // specialised on the pivot row's word pg (uniform), so that
-                // no per-lane word selection is needed
-                dispatch_word(pr >> 5, [&](auto Gc) {
-                    constexpr int pg = decltype(Gc)::value;
-                    uint32_t p = 0;
+                const int pg = pr >> 5;
+                const uint32_t pb = (uint32_t)pr & 31u;
+                // everything uniform is computed once per pivot: the pivot
+                // row's bit, f' = f without row pr and its multiples a^b f'
+                uint64_t wm[NH];
 #pragma unroll
-                    for (int d = 0; d < 4; ++d) {
-                        p |= ((f[d][pg] >> pb) & 1u) << d;
-                        f[d][pg] &= ~(1u << pb);
-                    }
-                    // a^b / p for the four bits of c[pr]
-                    const uint32_t sh = 4 * p;
-                    const uint32_t im[4] = {(uint32_t)(IM_TABLE[0] >> sh) & 0xFu, (uint32_t)(IM_TABLE[1] >> sh) & 0xFu,
-                                            (uint32_t)(IM_TABLE[2] >> sh) & 0xFu, (uint32_t)(IM_TABLE[3] >> sh) & 0xFu};
+                for (int h = 0; h < NH; ++h) wm[h] = h == (pr >> 6) ? 1ull << (pr & 63) : 0ull;
 #pragma unroll
-                    for (int s = 0; s < NCL; ++s) {
-                        // sets whose columns all precede pc need no update (pivot
-                        // columns are p e_pr, which the later steps leave alone;
-                        // free columns do not enter x), unless they hold y'
-                        if (s != Y_SET && set_rank(s) < ord) continue;
-                        uint32_t q = 0;
+                for (int d = 0; d < 4; ++d)
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) fw[d][h] &= ~wm[h];
+                uint32_t F[4][4][NG];  // F[b] = a^b f' (planes rotate; one 64-bit xor per pair)
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+                    uint64_t a0 = fw[0][h], a1 = fw[1][h], a2 = fw[2][h], a3 = fw[3][h];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const uint64_t a[4] = {a0, a1, a2, a3};
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
-                            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)cols[s][d][pg], (uint32_t)pb, 1u);
-                            q = gf16::axm(q, m, im[d]);
+                            F[b][d][2 * h] = (uint32_t)a[d];
+                            if (2 * h + 1 < NG) F[b][d][2 * h + 1] = (uint32_t)(a[d] >> 32);
                         }
-                        uint32_t qm[4];
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) qm[b] = gf16::bitmask(q, b);
-#pragma unroll
-                        for (int g = 0; g < NG; ++g) {
-                            uint32_t a0 = f[0][g], a1 = f[1][g], a2 = f[2][g], a3 = f[3][g];
-                            uint32_t c0 = cols[s][0][g], c1 = cols[s][1][g], c2 = cols[s][2][g], c3 = cols[s][3][g];
-#pragma unroll
-                            for (int b = 0; b < 4; ++b) {
-                                c0 = gf16::axm(c0, a0, qm[b]); c1 = gf16::axm(c1, a1, qm[b]);
-                                c2 = gf16::axm(c2, a2, qm[b]); c3 = gf16::axm(c3, a3, qm[b]);
-                                bs_mul_a(a0, a1, a2, a3);
-                            }
-                            cols[s][0][g] = c0; cols[s][1][g] = c1; cols[s][2][g] = c2; cols[s][3][g] = c3;
-                        }
+                        const uint64_t t = a3;
+                        a3 = a2; a2 = a1; a1 = a0 ^ t; a0 = t;
                     }
-                    S.pinv[pc] = (uint8_t)im[0];
-                });
-                // outside the specialised step, which would otherwise keep
-                // `used` in scratch memory
+                }
+                // bit masks of c[pr] for every column of set s: the pivot
+                // row's word picked by uniform selects (no branch), then a
+                // sign-extending one-bit extract
+                uint32_t wsel[NG];  // all ones for the pivot row's word
 #pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    if (g == (pr >> 5)) used[g] |= 1u << pb;
+                for (int g = 0; g < NG; ++g) wsel[g] = pg == g ? 0xFFFFFFFFu : 0u;
+                auto row_masks = [&](auto Sc, uint32_t (&cm)[4]) {
+                    constexpr int s = decltype(Sc)::value;
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        uint32_t w = cols[s][d][0];
+                        // w = wsel[g] ? word g : w, as v_bitop3 (a select
+                        // chain on a uniform index becomes an indexed
+                        // scratch load)
+#pragma unroll
+                        for (int g = 1; g < NG; ++g) w = __builtin_amdgcn_bitop3_b32(w, cols[s][d][g], wsel[g], 0xD8);
+                        cm[d] = (uint32_t)__builtin_amdgcn_sbfe((int)w, pb, 1u);
+                    }
+                };
+                // the pivot value p = f[pr], read from the pivot lane's masks,
+                // and a^b / p for the four bits of c[pr]
+                uint32_t cmp[4];
+                row_masks(std::integral_constant<int, ps>{}, cmp);
+                const uint32_t pv = (cmp[0] & 1u) | (cmp[1] & 2u) | (cmp[2] & 4u) | (cmp[3] & 8u);
+                const uint32_t sh = 4 * __builtin_amdgcn_readlane(pv, pl);
+                const uint32_t im[4] = {(uint32_t)(IM_TABLE[0] >> sh) & 0xFu, (uint32_t)(IM_TABLE[1] >> sh) & 0xFu,
+                                        (uint32_t)(IM_TABLE[2] >> sh) & 0xFu, (uint32_t)(IM_TABLE[3] >> sh) & 0xFu};
+                static_for<NCL>([&](auto Sc) {
+                    constexpr int s = decltype(Sc)::value;
+                    // sets whose columns all precede pc need no update (pivot
+                    // columns are p e_pr, which the later steps leave alone;
+                    // free columns do not enter x), unless they hold y'
+                    if constexpr (s != Y_SET && set_rank(s) < ord) return;
+                    uint32_t cm[4];
+                    if constexpr (s == ps) {
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) cm[d] = cmp[d];
+                    } else {
+                        row_masks(Sc, cm);
+                    }
+                    // q = c[pr] / p, and c += q f'
+                    uint32_t q = 0;
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) q = gf16::axm(q, cm[d], im[d]);
+                    uint32_t qm[4];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) qm[b] = (uint32_t)__builtin_amdgcn_sbfe((int)q, (uint32_t)b, 1u);
+#pragma unroll
+                    for (int g = 0; g < NG; ++g)
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            uint32_t c = cols[s][d][g];
+#pragma unroll
+                            for (int b = 0; b < 4; ++b) c = gf16::axm(c, F[b][d][g], qm[b]);
+                            cols[s][d][g] = c;
+                        }
+                });
+                if (lane == 0) S.pinv[pc] = (uint8_t)im[0];
+#pragma unroll
+                for (int h = 0; h < NH; ++h) used[h] |= wm[h];
                 ++rank;
             }
-            S.prow[pc] = (int16_t)pr;  // same value from every lane
+            if (lane == 0) S.prow[pc] = (int16_t)pr;
         }
-    }
+    });
     // x = r, plus y'[pivot row] / p on the pivot columns
     if (lane == Y_LANE) {
 #pragma unroll
