@@ -354,6 +354,12 @@ def main():
     ap.add_argument("--distinct-keys", type=int, default=65536,
                     help="items of the MAYO_2 distinct-key run (per-item key expansion fused with "
                          "signing and verification, 64-byte messages); 0 skips it")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time the headline sign and verify passes only (no graph, plain-form, reference-layout, "
+                         "serve, mixed-set, distinct-key, reference-flow or CPU legs): for profiler passes")
+    ap.add_argument("--launch", default="",
+                    help="launch configuration to use instead of autotuning, e.g. sign_prep=23,sign_solve=8,"
+                         "sign_chunks=2 (fields of cumayo_launch_config)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the compiled default launch variants (no launch autotuning)")
     ap.add_argument("--autotune-reps", type=int, default=3,
@@ -421,6 +427,9 @@ def main():
     # linear-algebra kernels timed on this batch, the fastest kept (untimed,
     # like the key preparation)
     tune = None
+    if args.launch:
+        eng.set_launch_config(**{k: int(x) for k, x in (kv.split("=") for kv in args.launch.split(","))})
+        args.no_autotune = True
     if not args.no_autotune:
         print("[bench] autotuning launch variants", file=sys.stderr, flush=True)
         tune = eng.autotune(p_esk, p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_s, ws_v,
@@ -452,61 +461,77 @@ def main():
     print(f"[bench] timed: sign {sign_s*1e3:.2f} ms/step verify {verify_s*1e3:.2f} ms/step",
           file=sys.stderr, flush=True)
 
-    # the same pipelines captured as HIP graphs (one graph launch per pass)
-    g_sigs = torch.zeros_like(d_sigs)
-    g_res = torch.full_like(d_res, 0xEE)
-    g_ws_s, g_ws_v = eng.sign_workspace(n), eng.verify_workspace(n)
-    g_sign = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, g_sigs, g_ws_s)
-    g_verify = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, g_res, g_ws_v)
-    g_sign.launch()
-    g_verify.launch()
-    torch.cuda.synchronize(dev)
-    graph_same = bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    comparisons = {}
+    if not args.headline_only:
+        # the same pipelines captured as HIP graphs (one graph launch per pass)
+        g_sigs = torch.zeros_like(d_sigs)
+        g_res = torch.full_like(d_res, 0xEE)
+        g_ws_s, g_ws_v = eng.sign_workspace(n), eng.verify_workspace(n)
+        g_sign = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, g_sigs, g_ws_s)
+        g_verify = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, g_res, g_ws_v)
         g_sign.launch()
-    torch.cuda.synchronize(dev)
-    graph_sign_s = (time.perf_counter() - t0) / args.steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
         g_verify.launch()
-    torch.cuda.synchronize(dev)
-    graph_verify_s = (time.perf_counter() - t0) / args.steps
-    graph_same = graph_same and bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
-    g_sign.close()
-    g_verify.close()
-    del g_sigs, g_res, g_ws_s, g_ws_v
-
-    # the plain form: one workgroup per item running the whole signing
-    # (or verification) stage in one kernel, the reference's own structure
-    def timed(fn, reps):
         torch.cuda.synchronize(dev)
+        graph_same = bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
         t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
+        for _ in range(args.steps):
+            g_sign.launch()
         torch.cuda.synchronize(dev)
-        return (time.perf_counter() - t0) / reps
+        graph_sign_s = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g_verify.launch()
+        torch.cuda.synchronize(dev)
+        graph_verify_s = (time.perf_counter() - t0) / args.steps
+        graph_same = graph_same and bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
+        g_sign.close()
+        g_verify.close()
+        del g_sigs, g_res, g_ws_s, g_ws_v
 
-    plain_sigs = torch.zeros_like(d_sigs)
-    plain_res = torch.full_like(d_res, 0xEE)
-    plain_sign = lambda: eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, plain_sigs, ws_s, stages=7)
-    plain_verify = lambda: eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, plain_res, ws_v,
-                                            stages=7)
-    plain_sign()
-    plain_verify()
-    plain_sign_s = timed(plain_sign, max(1, args.steps // 2))
-    plain_verify_s = timed(plain_verify, max(1, args.steps // 2))
-    plain_same = bool(torch.equal(plain_sigs, d_sigs)) and bool(torch.equal(plain_res, d_res))
-    # the same wave kernels over the reference-format key tables
-    ref_sigs = torch.zeros_like(d_sigs)
-    ref_res = torch.full_like(d_res, 0xEE)
-    ref_sign = lambda: eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, ref_sigs, ws_s)
-    ref_verify = lambda: eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, ref_res, ws_v)
-    ref_sign()
-    ref_verify()
-    ref_sign_s = timed(ref_sign, max(1, args.steps // 2))
-    ref_verify_s = timed(ref_verify, max(1, args.steps // 2))
-    ref_same = bool(torch.equal(ref_sigs, d_sigs)) and bool(torch.equal(ref_res, d_res))
+        # the plain form: one workgroup per item running the whole signing
+        # (or verification) stage in one kernel, the reference's own structure
+        def timed(fn, reps):
+            torch.cuda.synchronize(dev)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize(dev)
+            return (time.perf_counter() - t0) / reps
+
+        plain_sigs = torch.zeros_like(d_sigs)
+        plain_res = torch.full_like(d_res, 0xEE)
+        plain_sign = lambda: eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, plain_sigs, ws_s, stages=7)
+        plain_verify = lambda: eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, plain_res, ws_v,
+                                                stages=7)
+        plain_sign()
+        plain_verify()
+        plain_sign_s = timed(plain_sign, max(1, args.steps // 2))
+        plain_verify_s = timed(plain_verify, max(1, args.steps // 2))
+        plain_same = bool(torch.equal(plain_sigs, d_sigs)) and bool(torch.equal(plain_res, d_res))
+        # the same wave kernels over the reference-format key tables
+        ref_sigs = torch.zeros_like(d_sigs)
+        ref_res = torch.full_like(d_res, 0xEE)
+        ref_sign = lambda: eng.sign_table(d_esk, d_kidx, d_msgs, d_off, d_len, ref_sigs, ws_s)
+        ref_verify = lambda: eng.verify_table(d_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, ref_res, ws_v)
+        ref_sign()
+        ref_verify()
+        ref_sign_s = timed(ref_sign, max(1, args.steps // 2))
+        ref_verify_s = timed(ref_verify, max(1, args.steps // 2))
+        ref_same = bool(torch.equal(ref_sigs, d_sigs)) and bool(torch.equal(ref_res, d_res))
+        comparisons = {
+            "plain_form": {"kernels": "one workgroup per item, whole stage in one kernel",
+                           "sign_per_s": round(n / plain_sign_s, 1),
+                           "verify_per_s": round(n / plain_verify_s, 1),
+                           "speedup_sign": round(plain_sign_s / sign_s, 2),
+                           "speedup_verify": round(plain_verify_s / verify_s, 2),
+                           "identical_outputs": plain_same},
+            "reference_layout_tables": {"sign_per_s": round(n / ref_sign_s, 1),
+                                        "verify_per_s": round(n / ref_verify_s, 1),
+                                        "identical_outputs": ref_same},
+            "graph": {"pipelines": "sign and verify passes captured as HIP graphs, one graph launch each",
+                      "sign_per_s": round(n / graph_sign_s, 1), "verify_per_s": round(n / graph_verify_s, 1),
+                      "identical_outputs": graph_same},
+        }
     # restore the workspace of the default form for the attempt counts below
     sign_step()
     torch.cuda.synchronize(dev)
@@ -545,18 +570,7 @@ def main():
         "verify": {"value": round(verify_per_s, 1), "unit": "verify/s",
                    "ms_per_step": round(verify_s * 1e3, 3), "all_valid": all_verify,
                    "invalid": int(bad.size), "invalid_first": bad[:8].tolist()},
-        "plain_form": {"kernels": "one workgroup per item, whole stage in one kernel",
-                       "sign_per_s": round(n / plain_sign_s, 1),
-                       "verify_per_s": round(n / plain_verify_s, 1),
-                       "speedup_sign": round(plain_sign_s / sign_s, 2),
-                       "speedup_verify": round(plain_verify_s / verify_s, 2),
-                       "identical_outputs": plain_same},
-        "reference_layout_tables": {"sign_per_s": round(n / ref_sign_s, 1),
-                                    "verify_per_s": round(n / ref_verify_s, 1),
-                                    "identical_outputs": ref_same},
-        "graph": {"pipelines": "sign and verify passes captured as HIP graphs, one graph launch each",
-                  "sign_per_s": round(n / graph_sign_s, 1), "verify_per_s": round(n / graph_verify_s, 1),
-                  "identical_outputs": graph_same},
+        **comparisons,
         "launch_tuning": tune if tune is not None else {"best": eng.launch_config(), "autotuned": False},
         "sign_retries": {"items_with_retry": int((attempts > 1).sum()),
                          "exhausted": int((attempts == 0).sum()),
@@ -595,6 +609,9 @@ def main():
                 "clock_ghz": meas["clock_ghz"],
                 "frac_of_measured": round(achieved_tmacs / meas["gf16_tmacs"], 4)}
 
+    if args.headline_only:
+        args.flow_batch = args.serve_seconds = args.mixed_batch = args.distinct_keys = 0
+        args.no_cpu_baseline = True
     if args.flow_batch > 0:
         # the reference's own benchmark method through the drop-in C ABI
         # (a C++ program linked against the library): per-item key expansion

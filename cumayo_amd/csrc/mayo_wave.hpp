@@ -796,6 +796,42 @@ __device__ __forceinline__ void dispatch_word(int g, F &&f) {
     }
 }
 
+// o[d] = XOR over the set bits b of s of x[b][d], where x[b] holds the four
+// bit planes of a^b v (uniform 64-bit words): the product s v for a uniform
+// scalar s, as a switch over s so that each case is a fixed scalar xor
+// sequence (branch-free masking would take four and-xor pairs per plane).
+template <uint32_t SC>
+__device__ __forceinline__ void scale_case(const uint64_t (&x)[4][4], uint64_t (&o)[4]) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        uint64_t t = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if ((SC >> b) & 1u) t ^= x[b][d];
+        o[d] = t;
+    }
+}
+__device__ __forceinline__ void scale_planes(uint32_t sc, const uint64_t (&x)[4][4], uint64_t (&o)[4]) {
+    switch (sc) {
+        case 1: scale_case<1>(x, o); break;
+        case 2: scale_case<2>(x, o); break;
+        case 3: scale_case<3>(x, o); break;
+        case 4: scale_case<4>(x, o); break;
+        case 5: scale_case<5>(x, o); break;
+        case 6: scale_case<6>(x, o); break;
+        case 7: scale_case<7>(x, o); break;
+        case 8: scale_case<8>(x, o); break;
+        case 9: scale_case<9>(x, o); break;
+        case 10: scale_case<10>(x, o); break;
+        case 11: scale_case<11>(x, o); break;
+        case 12: scale_case<12>(x, o); break;
+        case 13: scale_case<13>(x, o); break;
+        case 14: scale_case<14>(x, o); break;
+        case 15: scale_case<15>(x, o); break;
+        default: scale_case<0>(x, o); break;
+    }
+}
+
 // f(integral_constant<int, i>) for i = I .. N-1, in order
 template <int N, int I = 0, class F>
 __device__ __forceinline__ void static_for(F &&f) {
@@ -903,22 +939,6 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                 for (int d = 0; d < 4; ++d)
 #pragma unroll
                     for (int h = 0; h < NH; ++h) fw[d][h] &= ~wm[h];
-                uint32_t F[4][4][NG];  // F[b] = a^b f' (planes rotate; one 64-bit xor per pair)
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    uint64_t a0 = fw[0][h], a1 = fw[1][h], a2 = fw[2][h], a3 = fw[3][h];
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const uint64_t a[4] = {a0, a1, a2, a3};
-#pragma unroll
-                        for (int d = 0; d < 4; ++d) {
-                            F[b][d][2 * h] = (uint32_t)a[d];
-                            if (2 * h + 1 < NG) F[b][d][2 * h + 1] = (uint32_t)(a[d] >> 32);
-                        }
-                        const uint64_t t = a3;
-                        a3 = a2; a2 = a1; a1 = a0 ^ t; a0 = t;
-                    }
-                }
                 // bit masks of c[pr] for every column of set s: the pivot
                 // row's word picked by uniform selects (no branch), then a
                 // sign-extending one-bit extract
@@ -938,14 +958,54 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                         cm[d] = (uint32_t)__builtin_amdgcn_sbfe((int)w, pb, 1u);
                     }
                 };
-                // the pivot value p = f[pr], read from the pivot lane's masks,
-                // and a^b / p for the four bits of c[pr]
+                // the pivot value p = f[pr], read from the pivot lane's masks
                 uint32_t cmp[4];
                 row_masks(std::integral_constant<int, ps>{}, cmp);
                 const uint32_t pv = (cmp[0] & 1u) | (cmp[1] & 2u) | (cmp[2] & 4u) | (cmp[3] & 8u);
-                const uint32_t sh = 4 * __builtin_amdgcn_readlane(pv, pl);
-                const uint32_t im[4] = {(uint32_t)(IM_TABLE[0] >> sh) & 0xFu, (uint32_t)(IM_TABLE[1] >> sh) & 0xFu,
-                                        (uint32_t)(IM_TABLE[2] >> sh) & 0xFu, (uint32_t)(IM_TABLE[3] >> sh) & 0xFu};
+                const uint32_t p = __builtin_amdgcn_readlane(pv, pl);
+                const uint32_t pinv = (uint32_t)(IM_TABLE[0] >> (4 * p)) & 0xFu;
+                // G[d] = a^d f' / p on the scalar unit (G[0] = p^-1 f' by a
+                // uniform switch over p^-1, then multiples by a), so that
+                // c += (c[pr] / p) f' is sum_d bit_d(c[pr]) G[d]: the masks of
+                // c[pr] are the update masks, with no per-lane division
+                uint64_t A[4][4][NH];  // A[b] = a^b f'
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) A[0][d][h] = fw[d][h];
+#pragma unroll
+                    for (int b = 1; b < 4; ++b) {
+                        A[b][0][h] = A[b - 1][3][h];
+                        A[b][1][h] = A[b - 1][0][h] ^ A[b - 1][3][h];
+                        A[b][2][h] = A[b - 1][1][h];
+                        A[b][3][h] = A[b - 1][2][h];
+                    }
+                }
+                // G[0] = p^-1 f' = sum of A[b] over the set bits of p^-1: four
+                // uniform branches, each a run of scalar xors
+                uint64_t G[4][4][NH];
+#pragma unroll
+                for (int d = 0; d < 4; ++d)
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) G[0][d][h] = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (__builtin_amdgcn_readfirstlane((pinv >> b) & 1u)) {
+#pragma unroll
+                        for (int d = 0; d < 4; ++d)
+#pragma unroll
+                            for (int h = 0; h < NH; ++h) G[0][d][h] ^= A[b][d][h];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < NH; ++h)
+#pragma unroll
+                    for (int b = 1; b < 4; ++b) {
+                        G[b][0][h] = G[b - 1][3][h];
+                        G[b][1][h] = G[b - 1][0][h] ^ G[b - 1][3][h];
+                        G[b][2][h] = G[b - 1][1][h];
+                        G[b][3][h] = G[b - 1][2][h];
+                    }
                 static_for<NCL>([&](auto Sc) {
                     constexpr int s = decltype(Sc)::value;
                     // sets whose columns all precede pc need no update (pivot
@@ -959,24 +1019,20 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                     } else {
                         row_masks(Sc, cm);
                     }
-                    // q = c[pr] / p, and c += q f'
-                    uint32_t q = 0;
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) q = gf16::axm(q, cm[d], im[d]);
-                    uint32_t qm[4];
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) qm[b] = (uint32_t)__builtin_amdgcn_sbfe((int)q, (uint32_t)b, 1u);
 #pragma unroll
                     for (int g = 0; g < NG; ++g)
 #pragma unroll
-                        for (int d = 0; d < 4; ++d) {
-                            uint32_t c = cols[s][d][g];
+                        for (int e = 0; e < 4; ++e) {
+                            uint32_t c = cols[s][e][g];
 #pragma unroll
-                            for (int b = 0; b < 4; ++b) c = gf16::axm(c, F[b][d][g], qm[b]);
-                            cols[s][d][g] = c;
+                            for (int d = 0; d < 4; ++d) {
+                                const uint64_t gw = G[d][e][g >> 1];
+                                c = gf16::axm(c, (g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw, cm[d]);
+                            }
+                            cols[s][e][g] = c;
                         }
                 });
-                if (lane == 0) S.pinv[pc] = (uint8_t)im[0];
+                if (lane == 0) S.pinv[pc] = (uint8_t)pinv;
 #pragma unroll
                 for (int h = 0; h < NH; ++h) used[h] |= wm[h];
                 ++rank;
