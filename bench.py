@@ -609,6 +609,13 @@ def main():
                 "clock_ghz": meas["clock_ghz"],
                 "frac_of_measured": round(achieved_tmacs / meas["gf16_tmacs"], 4)}
 
+    if not args.headline_only and os.path.exists(_build.contraction_path()):
+        # the shared-key P1 contraction in its bitsliced VALU form and its
+        # int8-MFMA GF(2)-plane form, same data, both checked against a CPU
+        # restatement (cumayo_amd/csrc/contraction_bench.hip)
+        cb = subprocess.run([_build.contraction_path()], capture_output=True, text=True, timeout=120)
+        if cb.returncode == 0 and cb.stdout.strip():
+            out["mfma_contraction"] = json.loads(cb.stdout.strip().splitlines()[-1])
     if args.headline_only:
         args.flow_batch = args.serve_seconds = args.mixed_batch = args.distinct_keys = 0
         args.no_cpu_baseline = True

@@ -79,14 +79,36 @@ def build_peak() -> str:
     return out
 
 
+CONTRACTION_SRC = os.path.join(HERE, "csrc", "contraction_bench.hip")
+
+
+def contraction_path() -> str:
+    return os.path.join(LIBDIR, "contraction_bench")
+
+
+def build_contraction() -> str:
+    """The P1-contraction measurement program: the bitsliced VALU column loop
+    against the int8-MFMA GF(2)-plane form of the same product (MAYO_1)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = contraction_path()
+    flags = [f for f in FLAGS if f not in ("-fPIC", "-shared")]
+    res = subprocess.run([HIPCC, *flags, "-DMAYO_VERSION=1", CONTRACTION_SRC, "-o", out + ".tmp"],
+                         capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for contraction_bench:\n{res.stderr[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(extra=()) -> list:
     def one(v):
         lib = build_one(v, extra)
         return [lib, build_flow(v)]
     with ThreadPoolExecutor(max_workers=4) as ex:
         fut = ex.submit(build_peak)
+        fut2 = ex.submit(build_contraction)
         built = [p for pair in ex.map(one, VERSIONS) for p in pair]
-        return built + [fut.result()]
+        return built + [fut.result(), fut2.result()]
 
 
 if __name__ == "__main__":
