@@ -75,6 +75,21 @@ void ensure_tables() {
 }
 
 inline unsigned blocks64(size_t n) { return (unsigned)((n + 63) / 64); }
+
+// SIMDs of the current device (compute units x 4), looked up once per device
+int device_simds() {
+    static std::atomic<int> simds[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    int v = simds[dev].load();
+    if (!v) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        v = 4 * cus;
+        simds[dev].store(v);
+    }
+    return v;
+}
 inline unsigned order_grid(size_t n) { return (unsigned)((n + ORDER_SEG - 1) / ORDER_SEG); }
 
 // Rows of `width` bytes from src (pitch spitch) to dst (pitch dpitch); one
@@ -505,7 +520,23 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             const int ordered = cfg.sign_order && n > 64;
             const int parts = cfg.sign_chunks > 1 && n >= CHUNK_MIN ? 2 : 1;
             if (ordered) k_key_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
-            if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
+            if (cfg.sign_chunks == 0) {
+                // one persistent fused launch (prepared or reference tables)
+                bool found = false;
+                with_occ<wv::PREP_OCCS>(std::is_same<KL, wv::PrepKeys>::value ? cfg.prep : wv::PREP_OCCS[0],
+                                        [&](auto P) {
+                    constexpr int code = decltype(P)::value;
+                    found = true;
+                    const size_t resident = (size_t)device_simds() * (code % 10);
+                    const unsigned grid = (unsigned)(n < resident ? n : resident);
+                    hipLaunchKernelGGL((wv::k_sign_fused<TableSign, KL, code>), dim3(grid), dim3(64), 0, stream,
+                                       acc, n, fail_first, ordered);
+                });
+                if (!found) {
+                    note(hipErrorInvalidValue);
+                    return (int)hipErrorInvalidValue;
+                }
+            } else if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
                 // prepared keys: the variants the launch autotuner chooses among
                 bool found = false;
                 with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto P) {
@@ -852,7 +883,7 @@ void cumayo_get_launch_config(cumayo_launch_config *out) {
 int cumayo_set_launch_config(const cumayo_launch_config *cfg) {
     if (!cfg || !listed(wv::PREP_OCCS, cfg->sign_prep) || !listed(wv::SOLVE_OCCS, cfg->sign_solve) ||
         !listed(wv::VERIFY_OCCS, cfg->verify) || (unsigned)cfg->sign_order > 1u ||
-        (unsigned)cfg->verify_order > 1u || cfg->sign_chunks < 1 || cfg->sign_chunks > 2)
+        (unsigned)cfg->verify_order > 1u || cfg->sign_chunks < 0 || cfg->sign_chunks > 2)
         return (int)hipErrorInvalidValue;
     g_prep.store(cfg->sign_prep);
     g_solve.store(cfg->sign_solve);
@@ -867,7 +898,7 @@ int cumayo_launch_variants(int field, int32_t *out, int max) {
     const int *v = nullptr;
     int cnt = 0;
     static const int orders[2] = {0, 1};
-    static const int chunks[2] = {1, 2};
+    static const int chunks[3] = {1, 2, 0};
     auto use = [&](const auto &arr) { v = arr, cnt = (int)(sizeof(arr) / sizeof(arr[0])); };
     switch (field) {
         case CUMAYO_FIELD_SIGN_PREP: use(wv::PREP_OCCS); break;
@@ -907,13 +938,15 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
         err = sign_table_impl<wv::PrepKeys>(acc, n, stream, 1, cfg);  // hashing pass, once
         double best = 1e30;
         LaunchCfg bc = cfg;
-        // the split into two parts only applies from CHUNK_MIN items on
+        // the split into two parts only applies from CHUNK_MIN items on;
+        // chunks 0 is the persistent fused kernel
         const int max_chunks = n >= CHUNK_MIN ? 2 : 1;
-        for (int chunks = 1; chunks <= max_chunks; ++chunks)
+        for (int chunks = 0; chunks <= max_chunks; ++chunks)
             for (int order = 0; order < 2; ++order)
                 for (int p : wv::PREP_OCCS)
                     for (int so : wv::SOLVE_OCCS) {
                         if (err) break;
+                        if (chunks == 0 && so != wv::SOLVE_OCCS[0]) continue;  // the fused kernel has no solve variant
                         LaunchCfg c = cfg;
                         c.prep = p, c.solve = so, c.sign_order = order, c.sign_chunks = chunks;
                         const double t = time_min_ms(

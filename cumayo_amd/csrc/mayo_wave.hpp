@@ -1150,6 +1150,48 @@ __global__ void __launch_bounds__(64, SIGN_WAVE_OCC) k_sign_wave(Acc acc, size_t
     wave_write_sig(S, it);
 }
 
+// The whole signing stage in one persistent launch: each wave takes items
+// k = blockIdx.x, + gridDim.x, ... (positions of the batch, or of the
+// XCD-aware order) and runs attempt 0 and any retries of each item itself:
+// quad form, M and y stay in LDS (no workspace hand-off), then the columns
+// of A, the elimination and the signature (sign.cu:244-478). One wave per
+// workgroup; the grid holds as many waves as the GPU keeps resident.
+template <bool PL>
+__device__ __noinline__ void retry_linear_algebra(Shared &S, int ctr, bool force_fail, int *state);
+
+template <class Acc, class KL, int CODE>
+__global__ void __launch_bounds__(64, CODE % 10) k_sign_fused(Acc acc, size_t n, int fail_first = 0,
+                                                              int ordered = 0) {
+    __shared__ Shared S;
+    __shared__ int state;  // 0 next attempt, 1 solved, 2 attempts exhausted
+    const int lane = threadIdx.x;
+    for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
+    for (size_t k = blockIdx.x; k < n; k += gridDim.x) {
+        SignItem it = acc(ordered ? acc.order(k) : k);
+        __syncthreads();
+        for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
+        for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
+        int ctr = 0;
+        for (;; ++ctr) {
+            __syncthreads();
+            wave_scol(S);
+            for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
+            __syncthreads();
+            quad_form<KL, O, false, 1, CODE / 10>(it.esk, S.scol, S.U, S.Mv);
+            __syncthreads();
+            // y, the columns of A and the elimination: a call of its own,
+            // which keeps the quadratic form's registers out of it
+            retry_linear_algebra<KL::PLANAR>(S, ctr, ctr < fail_first, &state);
+            __syncthreads();
+            if (state) break;
+            derive_vr<BlockSync>(S, it, ctr + 1);
+        }
+        if (lane == 0) it.scratch[SCR_ATTEMPTS] = state == 1 ? (uint8_t)(ctr + 1) : 0;
+        __syncthreads();
+        wave_write_sig(S, it);
+    }
+}
+
 // wave 0 of the retry kernel: y, the columns of A, the elimination (a call
 // of its own, which keeps the register use of the kernel down)
 template <bool PL>

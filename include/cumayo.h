@@ -346,11 +346,15 @@ typedef struct {
     int32_t sign_order;   /* 0 batch order, 1 XCD-aware key order */
     int32_t verify;       /* verification kernel (10 x rows per lane + waves per SIMD) */
     int32_t verify_order;
-    int32_t sign_chunks;  /* 1, or 2: from 16,384 items on, the signing linear-algebra pass
-                             runs as two halves on two streams (the second on a helper
-                             stream kept per calling thread and caller stream, forked from
-                             and joined back to the caller's stream), so that each half
-                             fills the other's kernel tails */
+    int32_t sign_chunks;  /* 1: the signing linear-algebra pass as three kernels over the
+                             batch (quadratic form + M + y, elimination, retries);
+                             2: from 16,384 items on, that pass as two halves on two
+                             streams (the second on a helper stream kept per calling thread
+                             and caller stream, forked from and joined back to the caller's
+                             stream), so that each half fills the other's kernel tails;
+                             0: one persistent fused kernel, each wave signing items in a
+                             grid-stride loop, retries included (sign_prep picks its
+                             variant; sign_solve does not apply) */
 } cumayo_launch_config;
 
 #define CUMAYO_FIELD_SIGN_PREP 0

@@ -81,7 +81,9 @@ def test_launch_variants_and_config(version):
         variants[name] = [buf[i] for i in range(cnt)]
         assert len(set(variants[name])) == cnt, (name, variants[name])
     assert lib.cumayo_launch_variants(99, None, 0) == -1
-    assert variants["order"] == [0, 1] and variants["sign_chunks"] == [1, 2]
+    # sign_chunks: three kernels over the batch (1, the default), that in two
+    # halves (2), one persistent fused kernel (0)
+    assert variants["order"] == [0, 1] and variants["sign_chunks"] == [1, 2, 0]
     cfg = LaunchConfig()
     lib.cumayo_get_launch_config(ctypes.byref(cfg))
     assert cfg.as_dict() == {"sign_prep": variants["sign_prep"][0], "sign_solve": variants["sign_solve"][0],
@@ -97,8 +99,9 @@ def test_launch_variants_and_config(version):
     assert got.as_dict() == alt.as_dict()
     bad = LaunchConfig(variants["sign_prep"][0] + 50, variants["sign_solve"][0], 0, variants["verify"][0], 0, 1)
     assert lib.cumayo_set_launch_config(ctypes.byref(bad)) != 0
-    bad = LaunchConfig(variants["sign_prep"][0], variants["sign_solve"][0], 0, variants["verify"][0], 0, 3)
-    assert lib.cumayo_set_launch_config(ctypes.byref(bad)) != 0
+    for chunks in (3, -1):
+        bad = LaunchConfig(variants["sign_prep"][0], variants["sign_solve"][0], 0, variants["verify"][0], 0, chunks)
+        assert lib.cumayo_set_launch_config(ctypes.byref(bad)) != 0
     lib.cumayo_get_launch_config(ctypes.byref(got))
     assert got.as_dict() == alt.as_dict()
     assert lib.cumayo_set_launch_config(ctypes.byref(cfg)) == 0

@@ -71,6 +71,27 @@ def test_every_launch_variant_gives_the_same_bytes(engine_factory, restore_confi
         torch.cuda.synchronize()
         assert torch.equal(out, ref), (p, s, order)
         assert np.array_equal(eng.sign_attempts(ws, n), ref_att), (p, s, order)
+    # the persistent fused kernel (sign_chunks 0) with every quadratic-form
+    # variant, also with every item's first two attempts forced to fail
+    for p, order in itertools.product(var["sign_prep"], var["order"]):
+        eng.set_launch_config(sign_prep=p, sign_order=order, sign_chunks=0)
+        out = torch.full_like(ref, 0xA5)
+        eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], out, ws)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), ("fused", p, order)
+        assert np.array_equal(eng.sign_attempts(ws, n), ref_att), ("fused", p, order)
+    eng.set_launch_config(sign_prep=var["sign_prep"][0], sign_order=0, sign_chunks=1)
+    forced = torch.zeros_like(ref)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], forced, ws, stages=3 | (2 << 8))
+    torch.cuda.synchronize()
+    forced_att = eng.sign_attempts(ws, n)
+    assert (forced_att >= 3).all()
+    eng.set_launch_config(sign_chunks=0)
+    out = torch.full_like(ref, 0xA5)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], out, ws, stages=3 | (2 << 8))
+    torch.cuda.synchronize()
+    assert torch.equal(out, forced) and np.array_equal(eng.sign_attempts(ws, n), forced_att)
+    eng.set_launch_config(sign_chunks=1)
     for vv, order in itertools.product(var["verify"], var["order"]):
         eng.set_launch_config(verify=vv, verify_order=order)
         res = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
@@ -107,7 +128,8 @@ def test_autotune_keeps_a_compiled_variant_and_the_results(engine_factory, resto
     assert best["sign_prep"] in var["sign_prep"] and best["sign_solve"] in var["sign_solve"]
     assert best["verify"] in var["verify"] and best["sign_order"] in (0, 1) and best["verify_order"] in (0, 1)
     chunk_options = 1 if n < 16384 else 2  # the two-part launch applies from 16,384 items on
-    assert rep["configs_timed"] == (2 * chunk_options * len(var["sign_prep"]) * len(var["sign_solve"]) +
+    # plus the persistent fused kernel, which has no solve variant
+    assert rep["configs_timed"] == (2 * len(var["sign_prep"]) * (1 + chunk_options * len(var["sign_solve"])) +
                                     2 * len(var["verify"]))
     assert 0 < rep["sign_ms"] <= rep["sign_default_ms"] and 0 < rep["verify_ms"] <= rep["verify_default_ms"]
     assert torch.equal(sigs, ref)
