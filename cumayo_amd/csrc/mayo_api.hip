@@ -145,6 +145,13 @@ void copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t wid
     note(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, stream));
 }
 
+// crypto_sign_signWithesk signs without L (1) or expands L and signs with it
+// as the reference does (0).
+#ifndef MAYO_SIGNW_LFREE
+#define MAYO_SIGNW_LFREE 1
+#endif
+constexpr int SIGNW_LFREE = MAYO_SIGNW_LFREE;
+
 // Pool offsets of the sign layouts (api.cu:97-104, 125-132).
 constexpr size_t SIGN_ESK_LEN = ESK_BYTES;
 constexpr size_t SIGNW_MSG_OFF = A128(ESK_BYTES);
@@ -153,7 +160,10 @@ constexpr size_t EPK_LEN = (size_t)P_BYTES + PK_BYTES - 16;
 constexpr size_t EPK_PAD = A128(EPK_LEN);
 constexpr size_t SIG_PAD = A128(SIG_BYTES);
 
-void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) {
+// expandsk (expandsk.cu:37-81): sk -> sk || O || P1 || L in place. With
+// with_L false the row keeps P2 where L would go (P1 || P2 is the keystream),
+// for the L-free signing kernel.
+void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream, bool with_L = true) {
     if (!n || !batch_ok(n)) return;
     ensure_tables();
     const size_t p1_off = SK + O_BYTES;
@@ -161,17 +171,22 @@ void launch_expandsk(uint8_t *pool, size_t pitch, size_t n, hipStream_t stream) 
     note_launch();
     launch_aes_expand(pool, pitch, n, p1_off, p1_off, stream);
     note_launch();
+    if (!with_L) return;
     wv::k_expand_L_wave<<<(unsigned)n, 64, 0, stream>>>(pool, pitch, n);
     note_launch();
 }
 
+// Signing over a reference-layout pool; lfree: the rows hold P2 in L's place.
 void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, size_t sig_off,
-                         size_t scr_off, size_t mlen, hipStream_t stream) {
+                         size_t scr_off, size_t mlen, hipStream_t stream, bool lfree = false) {
     if (!n || !batch_ok(n)) return;
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
-    wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+    if (lfree)
+        wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys, true><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+    else
+        wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }
 
@@ -302,9 +317,10 @@ void crypto_sign_signWithesk_HtD(uint8_t *csk, uint8_t *cMess, size_t mlen, uint
 
 void crypto_sign_signWithesk(uint8_t *pool, size_t Mlen, size_t pitch, size_t batch_size,
                              hipStream_t stream) {
-    launch_expandsk(pool, pitch, batch_size, stream);
+    // the key is expanded to sk || O || P1 || P2 and signed without L
+    launch_expandsk(pool, pitch, batch_size, stream, SIGNW_LFREE == 0);
     const size_t msg_off = SIGNW_MSG_OFF, sig_off = msg_off + A128(Mlen), scr_off = sig_off + SIG_PAD;
-    launch_sign_pitched(pool, pitch, batch_size, msg_off, sig_off, scr_off, Mlen, stream);
+    launch_sign_pitched(pool, pitch, batch_size, msg_off, sig_off, scr_off, Mlen, stream, SIGNW_LFREE != 0);
 }
 
 void crypto_sign_signWithesk_DtH(uint8_t *sig, uint8_t *pool, size_t Mlen, size_t pitch,

@@ -93,6 +93,15 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | (uint64_t)uni((uint32_t)x);
 }
 
+// f(integral_constant<int, i>) for i = I .. N-1, in order
+template <int N, int I = 0, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
 // Scalar-unit ops on wave-uniform values. Written out so that the compiler
 // keeps them on the SALU: left to itself it fuses and-not / xor chains of
 // uniform values into v_bitop3 on the vector unit, which is the busier port
@@ -278,6 +287,20 @@ struct EntryLoad {
     }
 };
 
+// ---- the oil space O, packed by rows ---------------------------------------------
+constexpr int OWD = (O + 7) / 8;            // words of one packed row of O
+
+// ocol[c * OWD + q]: nibbles x = 8q .. 8q+7 of row c of O.
+__device__ __forceinline__ void load_ocol(const uint8_t *Onib, uint32_t *ocol) {
+    for (int c = threadIdx.x; c < V; c += 64)
+#pragma unroll
+        for (int q = 0; q < OWD; ++q) {
+            uint32_t v = 0;
+            for (int x = 8 * q; x < O && x < 8 * q + 8; ++x) v |= mvec::nib(Onib, c * O + x) << (4 * (x - 8 * q));
+            ocol[c * OWD + q] = v;
+        }
+}
+
 // ---- quadratic form -------------------------------------------------------------
 // T_j[row] = sum_c P[row][c] S_j[c] for every row a lane owns, with c uniform
 // across the wave. Writing S_j[c] = sum_b s_jb(c) a^b (bits s_jb), T_j =
@@ -298,13 +321,25 @@ struct EntryLoad {
 // Lanes = (word w, row slot); a chunk gives each slot RT rows, rows in
 // ascending start column so that every row of a chunk starts within a few
 // columns of the first (those columns are masked per lane).
+//
+// LF (signing without L, from a key row that holds P2 where L would be):
+// the first NLR rows read P2[c][x], which gives the P2 V_j part of M_j[x],
+// and the rest, sum_r O[r][x] W_j[r] with W_j = (P1 + P1^T) S_j, is added
+// per chunk. Once a P1 row's T_j[r] has gone into U, the row goes on over
+// the entries P1[c][r], c <= r (the column r of P1; the diagonal, counted
+// twice, cancels), so its accumulators then hold W_j[r]; the chunk's W rows
+// go to Wc (LDS, RT SL K MW words), and lanes (j, w) fold them into Mv with
+// the SGPR masks of O[r][x] (ocol, packed rows of O). One wave only.
+// Costs K (P1 entries + V O) MACs per signature, against V^2 O for L.
 template <int NLR, bool FULL, int RT = QF_RT>
 constexpr int qf_chunks() {
     return (NLR + (FULL ? N : V) + RT * SL - 1) / (RT * SL);
 }
-template <class KL, int NLR, bool FULL, int NW = 1, int RT = QF_RT>
+template <class KL, int NLR, bool FULL, int NW = 1, int RT = QF_RT, bool LF = false>
 __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *scol, uint32_t *Ulds,
-                                          uint32_t *Mv) {
+                                          uint32_t *Mv, const uint32_t *ocol = nullptr,
+                                          uint32_t *Wc = nullptr) {
+    static_assert(!LF || (NW == 1 && !FULL && NLR == O), "the L-free form is one wave's signing");
     constexpr int NP = FULL ? N : V;          // P rows and columns
     constexpr int NVR = NLR + NP;             // rows of the schedule
     constexpr int RPC = RT * SL;           // rows per chunk
@@ -496,6 +531,106 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
                                          am[a][3]);
                     }
                 }
+            }
+        }
+        if constexpr (LF) {
+            // P1 rows of the chunk: r_first .. r_last
+            const int vp = v0 > NLR ? v0 : NLR;
+            if (vp <= vlast) {
+                const int r_first = vp - NLR, r_last = vlast - NLR;
+                int rl[RT];
+                uint32_t lbase[RT];
+#pragma unroll
+                for (int tt = 0; tt < RT; ++tt) {
+                    const int vr = v0 + tt * SL + slot;
+                    rl[tt] = active && vr >= NLR && vr <= vlast ? vr - NLR : -1;
+                    // entry (c, r) = tri(c, c) - c + r; rows without a P1 row
+                    // read (c, 0)'s address (in P1) and are not used
+                    lbase[tt] = P1o + (uint32_t)((rl[tt] > 0 ? rl[tt] : 0) * EB) + 4 * w;
+                }
+                constexpr int NSET = KL::AHEAD;
+                EntryLoad<KL> ld[NSET][RT];
+                auto lfetch = [&](auto S, int c) {
+                    const uint32_t lb = (uint32_t)((c * (V - 1) - c * (c - 1) / 2) * EB);
+#pragma unroll
+                    for (int tt = 0; tt < RT; ++tt) ld[decltype(S)::value][tt].fetch(key, lbase[tt] + lb, nv);
+                };
+                // columns 0 .. r_last; below r_first every row of the chunk
+                // takes the entry, from r_first on only rows r >= c
+                static_for<NSET>([&](auto S) {
+                    if (decltype(S)::value <= r_last) lfetch(S, decltype(S)::value);
+                });
+#pragma unroll 1
+                for (int c0 = 0; c0 <= r_last; c0 += NSET) {
+                    static_for<NSET>([&](auto S) {
+                        constexpr int s = decltype(S)::value;
+                        const int c = c0 + s;
+                        if (c <= r_last) {
+                            const uint64_t sc = uni64(scol[c]);
+                            const uint32_t sh[2] = {(uint32_t)sc, (uint32_t)(sc >> 32)};
+                            uint32_t x[RT];
+#pragma unroll
+                            for (int tt = 0; tt < RT; ++tt) x[tt] = ld[s][tt].word(nv);
+                            if (c >= r_first) {
+#pragma unroll
+                                for (int tt = 0; tt < RT; ++tt) x[tt] = c <= rl[tt] ? x[tt] : 0u;
+                            }
+                            if (c + NSET <= r_last) lfetch(S, c + NSET);
+#pragma unroll
+                            for (int j = 0; j < K; ++j)
+#pragma unroll
+                                for (int b = 0; b < 4; ++b) {
+                                    const int bit = 4 * j + b;
+                                    const uint32_t m = gf16::bitmask(sh[bit >> 5], bit & 31);
+#pragma unroll
+                                    for (int tt = 0; tt < RT; ++tt) Tb[tt][j][b] = gf16::axm(Tb[tt][j][b], x[tt], m);
+                                }
+                        }
+                    });
+                }
+                // W_j[r] of the chunk's rows into Wc (row index vr - v0)
+#pragma unroll
+                for (int tt = 0; tt < RT; ++tt) {
+                    if (rl[tt] < 0) continue;
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        uint32_t Wj;
+                        if constexpr (KL::PLANAR) {
+                            using namespace gf16::planar;
+                            Wj = mul_a_add(mul_a_add(mul_a_add(Tb[tt][j][3], Tb[tt][j][2]), Tb[tt][j][1]), Tb[tt][j][0]);
+                        } else {
+                            Wj = gf16::mul_a(gf16::mul_a(gf16::mul_a(Tb[tt][j][3]) ^ Tb[tt][j][2]) ^ Tb[tt][j][1]) ^
+                                 Tb[tt][j][0];
+                        }
+                        Wc[((tt * SL + slot) * K + j) * MW + w] = Wj;
+                    }
+                }
+                WaveSync::sync();
+                // M_j[x] += sum_r O[r][x] W_j[r] over the chunk's rows
+                for (int it = lane; it < K * MW; it += 64) {
+                    const int j = it / MW, ww = it % MW;
+                    uint32_t acc[O];
+#pragma unroll
+                    for (int x = 0; x < O; ++x) acc[x] = 0;
+#pragma unroll 1
+                    for (int vr = vp; vr <= vlast; ++vr) {
+                        const int r = vr - NLR;
+                        const std::conditional_t<KL::PLANAR, gf16::planar::Multiples, gf16::Multiples> mt(
+                            Wc[((vr - v0) * K + j) * MW + ww]);
+                        uint32_t oc[OWD];
+#pragma unroll
+                        for (int q = 0; q < OWD; ++q) oc[q] = uni(ocol[r * OWD + q]);
+#pragma unroll
+                        for (int x = 0; x < O; ++x) {
+                            const uint32_t ox = oc[x >> 3] >> (4 * (x & 7));
+                            acc[x] = gf16::mac(acc[x], mt.x0, mt.x1, mt.x2, mt.x3, gf16::bitmask(ox, 0),
+                                               gf16::bitmask(ox, 1), gf16::bitmask(ox, 2), gf16::bitmask(ox, 3));
+                        }
+                    }
+#pragma unroll
+                    for (int x = 0; x < O; ++x) Mv[(j * O + x) * MW + ww] ^= acc[x];
+                }
+                WaveSync::sync();  // Wc is rewritten by the next chunk
             }
         }
     }
@@ -832,15 +967,6 @@ __device__ __forceinline__ void scale_planes(uint32_t sc, const uint64_t (&x)[4]
     }
 }
 
-// f(integral_constant<int, i>) for i = I .. N-1, in order
-template <int N, int I = 0, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>{});
-        static_for<N, I + 1>(f);
-    }
-}
-
 // Gauss-Jordan on the columns of [A | y - A r]; x (S.x) = r + z. Returns 1
 // on full rank. Each pivot step: the pivot column f is broadcast (SGPRs),
 // its pivot row pr is the lowest unused row with a nonzero entry, p = f[pr];
@@ -1120,11 +1246,16 @@ constexpr int SIGN_WAVE_OCC = M > 80 && M <= 108 ? 2 : QF_OCC;
 
 // The whole signing loop in one wave (sign.cu:244-478), for the reference
 // pool. Attempts below fail_first report a singular system (test hook).
-template <class Acc, class KL = RefKeys>
+// LF: the key row holds P2 where the expanded key has L, and M is formed
+// without L (the L-free quadratic form above).
+template <class Acc, class KL = RefKeys, bool LF = false>
 __global__ void __launch_bounds__(64, SIGN_WAVE_OCC) k_sign_wave(Acc acc, size_t n, int fail_first = 0) {
     __shared__ Shared S;
+    __shared__ uint32_t ocol[LF ? V * OWD : 1];
+    __shared__ uint32_t Wc[LF ? QF_RT * SL * K * MW : 1];
     const int lane = threadIdx.x;
     SignItem it = acc(blockIdx.x);
+    if constexpr (LF) load_ocol(it.esk + SK, ocol);
     for (int w = lane; w < MW; w += 64) S.t[w] = mvec::load_mw(it.scratch + SCR_T, w);
     for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
@@ -1134,7 +1265,7 @@ __global__ void __launch_bounds__(64, SIGN_WAVE_OCC) k_sign_wave(Acc acc, size_t
         wave_scol(S);
         for (int e = lane; e < PAIRS * MW; e += 64) S.U[e] = 0;
         __syncthreads();
-        quad_form<KL, O, false>(it.esk, S.scol, S.U, S.Mv);
+        quad_form<KL, O, false, 1, QF_RT, LF>(it.esk, S.scol, S.U, S.Mv, ocol, Wc);
         __syncthreads();
         wave_y(S, S.t);
         planes_of_M<KL::PLANAR>(S.Mv, S.Mb);  // Mb overlays U, which wave_y has consumed
@@ -1319,19 +1450,7 @@ __global__ void __launch_bounds__(64, OCC) k_sign_solve(Acc acc, size_t n, int f
 // rows of a chunk, columns left of the chunk read lower entries and columns
 // right of it upper entries at lane-constant offsets; only the chunk's own
 // columns need a per-lane choice. Entry words are fetched one column ahead.
-constexpr int OWD = (O + 7) / 8;            // words of one packed row of O
 constexpr int KRT = O <= 12 ? 2 : 1;        // rows per lane per chunk
-
-// ocol[c * OWD + q]: nibbles x = 8q .. 8q+7 of row c of O.
-__device__ __forceinline__ void load_ocol(const uint8_t *Onib, uint32_t *ocol) {
-    for (int c = threadIdx.x; c < V; c += 64)
-#pragma unroll
-        for (int q = 0; q < OWD; ++q) {
-            uint32_t v = 0;
-            for (int x = 8 * q; x < O && x < 8 * q + 8; ++x) v |= mvec::nib(Onib, c * O + x) << (4 * (x - 8 * q));
-            ocol[c * OWD + q] = v;
-        }
-}
 
 // row_done(r, T) is called for each valid row of the lane with T[x] = T_x[r]
 // (this lane's word w). Entry words are loaded PF columns ahead.

@@ -74,7 +74,9 @@ void crypto_sign_sign(uint8_t *d_sign_mem_pool, size_t Mlen, size_t pitch, size_
 void crypto_sign_sign_DtH(uint8_t *sig, uint8_t *d_sig_mem_pool, size_t Mlen, size_t pitch,
                           size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
 /* api.cuh:31-35 (api.cu:116-144): compact sk in, expanded then signed;
- * rows A128(esk) || A128(msg) || A128(sig) || scratch */
+ * rows A128(esk) || A128(msg) || A128(sig) || scratch. Signs without L: after
+ * the call the esk region holds sk || O || P1 || P2 (the reference leaves
+ * sk || O || P1 || L there); the signatures are the reference's. */
 void crypto_sign_signWithesk_HtD(uint8_t *cesk, uint8_t *cMess, size_t mlen,
                                  uint8_t *d_sign_mem_pool, size_t pitch, size_t batch_size,
                                  hipStream_t stream CUMAYO_DEFAULT(nullptr));

@@ -1,0 +1,99 @@
+"""crypto_sign_signWithesk signs without L: the pool rows keep P2 where the
+reference's expansion writes L = (P1 + P1^T) O + P2 (expandsk.cu:37-81), and
+the signing kernel forms M_j = V_j^T P2 + (W_j)^T O from W_j = (P1 + P1^T)
+V_j instead of V_j^T L (sign.cu:256-330).
+
+* every parameter set: distinct compact keys, one message each, the L-free
+  signatures equal, item by item, those of crypto_sign_expandsk followed by
+  crypto_sign_sign (which expand and use L) over the same keys and
+  messages, and a sample equals the oracle's;
+* the two natural second attempts of the MAYO_1 retry workload (a rank
+  failure on attempt 0, then a new V from the attempt counter) through the
+  L-free path, against the oracle;
+* the pool after the call holds P1 || P2 in the esk's P1 || L place.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = {1: 2048, 2: 8192, 3: 1024, 5: 512}
+
+
+def messages(n, mlen, tag):
+    import hashlib
+    return np.frombuffer(hashlib.shake_256(f"lfree/{tag}".encode()).digest(n * mlen),
+                         np.uint8).reshape(n, mlen).copy()
+
+
+@pytest.mark.parametrize("v", [1, 2, 3, 5], ids=lambda v: f"MAYO_{v}")
+def test_lfree_equals_L_path(engine_factory, v):
+    n = SIZES[v]
+    eng = engine_factory(v)
+    print(f"[lfree] MAYO_{v} n={n}", flush=True)
+    pk, sk = eng.keypair(n, rand_index=3)
+    msgs = messages(n, 48, f"eq{v}")
+    sig = eng.sign_with_sk(sk, msgs)
+    esk = eng.expandsk(sk)
+    sig_L = eng.sign(esk, msgs)
+    bad = np.where((sig != sig_L).any(axis=1))[0]
+    assert bad.size == 0, f"MAYO_{v}: items {bad[:10].tolist()} differ from the L path"
+    for i in list(range(0, n, n // 8)) + [n - 1]:
+        osk = sk[i].tobytes()
+        oesk = O.expandsk(v, osk)
+        assert esk[i].tobytes() == oesk, f"expandsk {i}"
+        osig, _ = O.sign(v, oesk, msgs[i].tobytes())
+        assert sig[i].tobytes() == osig, f"sign {i}"
+    res = eng.verify_with_pk(pk, msgs, sig)
+    assert int((res != 0).sum()) == 0
+
+
+def test_lfree_natural_retries_mayo1(engine_factory):
+    """Items 2768 and 6420 of the bench generator's "retry-test" workload
+    (16 keys, key j mod 16, lengths in [32, 1024]) need a second attempt;
+    each is signed here with its compact key through crypto_sign_signWithesk."""
+    import bench
+    v = 1
+    eng = engine_factory(v)
+    pk, sk = eng.keypair(16)
+    flat, offs, lens, kidx = bench.make_workload(8192, 16, 32, 1024, tag="retry-test")
+    flat = np.asarray(flat)
+    for i in (2768, 6420):
+        k = int(kidx[i])
+        m = flat[offs[i]:offs[i] + lens[i]].copy()
+        print(f"[lfree] retry item {i} key {k} len {lens[i]}", flush=True)
+        sig = eng.sign_with_sk(sk[k:k + 1], m.reshape(1, -1))
+        osig, oatt = O.sign(v, O.expandsk(v, sk[k].tobytes()), m.tobytes())
+        assert oatt == 2, f"item {i}: the oracle takes {oatt} attempts"
+        assert sig[0].tobytes() == osig, f"item {i}"
+
+
+def test_lfree_pool_keeps_P2(engine_factory):
+    """After crypto_sign_signWithesk a row holds sk || O || P1 || P2 (the
+    expanded key with P2 in L's place), then the message and signature."""
+    from cumayo_amd.engine import a128
+    v, n, mlen = 2, 4, 40
+    eng = engine_factory(v)
+    z = eng.sizes
+    pk, sk = eng.keypair(n, rand_index=9)
+    msgs = messages(n, mlen, "pool")
+    row = a128(z.esk_bytes) + a128(mlen) + a128(z.sig_bytes) + a128(z.scratch_bytes)
+    pool, pitch = eng.pool(n, row)
+    h_sk, h_msg = eng.pinned(sk.shape, sk), eng.pinned(msgs.shape, msgs)
+    st = eng.stream_handle()
+    eng.lib.crypto_sign_signWithesk_HtD(h_sk.data_ptr(), h_msg.data_ptr(), mlen, pool.data_ptr(), pitch, n, st)
+    eng.lib.crypto_sign_signWithesk(pool.data_ptr(), mlen, pitch, n, st)
+    eng.torch.cuda.synchronize(eng.device)
+    eng.check()
+    rows = pool.cpu().numpy()
+    p1_off = z.sk_bytes + z.o_bytes
+    for i in range(n):
+        oesk = O.expandsk(v, sk[i].tobytes())
+        # sk || O || P1 as expanded, then P2 (that of the expanded public key)
+        assert rows[i, :p1_off + z.p1_bytes].tobytes() == oesk[:p1_off + z.p1_bytes], f"row {i}"
+        p2 = O.expandpk(v, pk[i].tobytes())[z.p1_bytes:z.p_bytes]
+        assert rows[i, p1_off + z.p1_bytes:p1_off + z.p_bytes].tobytes() == p2, f"row {i} P2"
+        assert oesk[p1_off + z.p1_bytes:p1_off + z.p_bytes] != p2  # L differs from P2
+        assert rows[i, a128(z.esk_bytes):a128(z.esk_bytes) + mlen].tobytes() == msgs[i].tobytes()
