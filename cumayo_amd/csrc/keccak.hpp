@@ -164,6 +164,75 @@ struct Shake {
         st[RW - 1] ^= 0x8000000000000000ull;
         f1600(st);
     }
+    // ---- register-only forms: every state index is a compile-time value, so
+    // the state stays in VGPRs (the dynamic-position helpers above put the
+    // rate words on the stack) ----
+
+    // Absorb the len-byte message at p into a fresh state (position 0): full
+    // blocks, then the tail word by word (bytes past the end are neither
+    // read nor absorbed). Returns the tail length, the position to finalize at.
+    __device__ __forceinline__ int absorb_msg(const uint8_t *p, size_t len) {
+        while (len >= (size_t)RATE) {
+#pragma unroll
+            for (int i = 0; i < RW; ++i) st[i] ^= load64(p + 8 * i);
+            f1600(st);
+            p += RATE; len -= RATE;
+        }
+        const int tail = (int)len;
+#pragma unroll
+        for (int w = 0; w < RW; ++w) {
+            const int nb = tail - 8 * w;
+            uint64_t x = 0;
+            if (nb >= 8) {
+                x = load64(p + 8 * w);
+            } else if (nb > 0) {
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    if (b < nb) x |= (uint64_t)p[8 * w + b] << (8 * b);
+            }
+            st[w] ^= x;
+        }
+        return tail;
+    }
+    // Pad at position pos (< RATE, a runtime value) and permute.
+    __device__ __forceinline__ void finalize_at(int pos) {
+#pragma unroll
+        for (int w = 0; w < RW; ++w) st[w] ^= w == (pos >> 3) ? 0x1Full << (8 * (pos & 7)) : 0ull;
+        st[RW - 1] ^= 0x8000000000000000ull;
+        f1600(st);
+    }
+    // Absorb LEN bytes from p as the input bytes [POS, POS + LEN) (the
+    // stream so far being POS bytes long), permuting after each full block.
+    template <int POS, int LEN>
+    __device__ __forceinline__ void absorb_at(const uint8_t *p) {
+#pragma unroll
+        for (int w = POS / 8; w <= (POS + LEN - 1) / 8; ++w) {
+            uint64_t x = 0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int at = 8 * w + b;
+                if (at >= POS && at < POS + LEN) x |= (uint64_t)p[at - POS] << (8 * b);
+            }
+            st[w % RW] ^= x;
+            if (w % RW == RW - 1 && 8 * w + 7 < POS + LEN) f1600(st);
+        }
+    }
+    // One byte b at stream position POS.
+    template <int POS>
+    __device__ __forceinline__ void absorb_byte_at(uint32_t b) {
+        constexpr int at = POS % RATE;
+        st[at >> 3] ^= (uint64_t)(b & 0xFF) << (8 * (at & 7));
+        if constexpr (at == RATE - 1) f1600(st);
+    }
+    // Pad after a POS-byte stream and permute.
+    template <int POS>
+    __device__ __forceinline__ void finalize_static() {
+        constexpr int at = POS % RATE;
+        st[at >> 3] ^= 0x1Full << (8 * (at & 7));
+        st[RW - 1] ^= 0x8000000000000000ull;
+        f1600(st);
+    }
+
     // Squeeze len bytes (a single call after finalize) into out.
     __device__ void squeeze(uint8_t *out, int len) {
         for (int base = 0;; base += RATE) {

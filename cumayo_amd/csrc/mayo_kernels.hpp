@@ -162,14 +162,14 @@ __global__ void __launch_bounds__(64) k_seed(uint8_t *pool, size_t pitch, size_t
         uint64_t ctr = rand_index + 2 * (uint64_t)i + 1;
         keccak::Shake<168> h;
         h.init();
-        for (int b = 0; b < 8; ++b) h.absorb_byte((uint32_t)(ctr >> (8 * b)) & 0xFF);
-        h.finalize();
+        h.st[0] ^= ctr;  // LE64(ctr), the whole input
+        h.finalize_static<8>();
         h.squeeze(sk, SK);
     }
     keccak::Shake<136> h;
     h.init();
-    h.absorb(sk, SK);
-    h.finalize();
+    h.absorb_at<0, SK>(sk);
+    h.finalize_static<SK>();
     // seed_pk (16 bytes) and O (O_BYTES) may go to different places
     uint8_t *seed = row + seed_off, *ob = row + o_off;
     for (int base = 0;; base += 136) {
@@ -320,24 +320,26 @@ __global__ void __launch_bounds__(64) k_sign_hash(Acc acc, size_t n) {
     uint8_t *s = it.scratch;
     keccak::Shake<136> h;
     h.init();
-    h.absorb(it.msg, it.mlen);
-    h.finalize();
+    h.finalize_at(h.absorb_msg(it.msg, it.mlen));
     h.squeeze(s + SCR_DIGEST, DIGEST);
+    // salt = SHAKE256(digest || R || seed_sk), R = 0 (absorbing zeros is a
+    // no-op; no block ends inside R, so no permutation is skipped with it)
+    static_assert((DIGEST + SK) <= 136, "R lies in the first block");
     h.init();
-    h.absorb(s + SCR_DIGEST, DIGEST);
-    for (int b = 0; b < SK; ++b) h.absorb_byte(0);  // R = 0
-    h.absorb(it.esk, SK);                            // seed_sk
-    h.finalize();
+    h.absorb_at<0, DIGEST>(s + SCR_DIGEST);
+    h.absorb_at<DIGEST + SK, SK>(it.esk);
+    h.finalize_static<DIGEST + 2 * SK>();
     h.squeeze(s + SCR_SALT, SK);
+    // t = SHAKE256(digest || salt)
     h.init();
-    h.absorb(s + SCR_DIGEST, DIGEST + SK);
-    h.finalize();
+    h.absorb_at<0, DIGEST + SK>(s + SCR_DIGEST);
+    h.finalize_static<DIGEST + SK>();
     h.squeeze(s + SCR_T, MB);
+    // V || r = SHAKE256(digest || salt || seed_sk || ctr), ctr = 0
     h.init();
-    h.absorb(s + SCR_DIGEST, DIGEST + SK);
-    h.absorb(it.esk, SK);
-    h.absorb_byte(0);  // ctr = 0
-    h.finalize();
+    h.absorb_at<0, DIGEST + SK>(s + SCR_DIGEST);
+    h.absorb_at<DIGEST + SK, SK>(it.esk);
+    h.finalize_static<DIGEST + 2 * SK + 1>();
     h.squeeze(s + SCR_VR, VR_BYTES);
 }
 
@@ -347,13 +349,12 @@ __device__ __forceinline__ void verify_target(const uint8_t *msg, uint32_t mlen,
                                               uint8_t *t) {
     keccak::Shake<136> h;
     h.init();
-    h.absorb(msg, mlen);
-    h.finalize();
+    h.finalize_at(h.absorb_msg(msg, mlen));
     h.squeeze(digest_tmp, DIGEST);
     h.init();
-    h.absorb(digest_tmp, DIGEST);
-    h.absorb(sig + SIG_S_BYTES, SK);  // salt
-    h.finalize();
+    h.absorb_at<0, DIGEST>(digest_tmp);
+    h.absorb_at<DIGEST, SK>(sig + SIG_S_BYTES);  // salt
+    h.finalize_static<DIGEST + SK>();
     h.squeeze(t, MB);
 }
 

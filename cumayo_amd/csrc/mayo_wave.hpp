@@ -93,6 +93,16 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | (uint64_t)uni((uint32_t)x);
 }
 
+// threadIdx.x behind an empty asm statement. The phases of the whole-loop
+// signing kernels derive per-lane constants from it; taken this way inside
+// each phase they are recomputed there, instead of being hoisted out of the
+// attempt loop and held (spilled) across the quadratic form.
+__device__ __forceinline__ int lane_here() {
+    int l = (int)threadIdx.x;
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 // f(integral_constant<int, i>) for i = I .. N-1, in order
 template <int N, int I = 0, class F>
 __device__ __forceinline__ void static_for(F &&f) {
@@ -347,7 +357,7 @@ __device__ __forceinline__ void quad_form(const uint8_t *key, const uint64_t *sc
     constexpr int EB = KL::EB;
     constexpr uint32_t P1o = FULL ? KL::EPK_P1 : KL::ESK_P1;
     // NW waves of the workgroup share the chunks (chunk ch goes to wave ch % NW)
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_here() & 63;
     const int wave = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int w = lane % MW, slot = lane / MW;
     const bool active = slot < SL;
@@ -712,7 +722,7 @@ constexpr int YG = YG_MAX;
 constexpr int YPG = (PAIRS + YG - 1) / YG;
 template <class Sync>
 __device__ __forceinline__ void unreduced_y(const uint32_t *U, uint32_t *ypart) {
-    const int lane = threadIdx.x & 63, q = lane % UW, g = lane / UW;
+    const int lane = lane_here() & 63, q = lane % UW, g = lane / UW;
     if (g < YG) {
         // first pair (i, j) of the group
         int p = g * YPG, i = 0;
@@ -743,7 +753,7 @@ __device__ __forceinline__ void unreduced_y(const uint32_t *U, uint32_t *ypart) 
 // y = t + sum_l E^l u_l computed by the wave; result in S.y (MW words).
 template <class Sync = BlockSync>
 __device__ __forceinline__ void wave_y(Shared &S, const uint32_t *t) {
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_here() & 63;
     unreduced_y<Sync>(S.U, S.yu);
     if (lane == 0) {
         uint32_t acc[UW], out[MW];
@@ -780,7 +790,7 @@ struct SolveShared {
 // Mb[(v*4 + d)*NG + g] holds bit d of rows 32g .. 32g+31.
 template <bool PL, class SRC>
 __device__ __forceinline__ void planes_of_M(const SRC *src, uint32_t *Mb) {
-    for (int it = threadIdx.x; it < KO * NG; it += 64) {
+    for (int it = lane_here(); it < KO * NG; it += 64) {
         const int v = it / NG, g = it % NG;
         uint32_t w[4];
 #pragma unroll
@@ -819,7 +829,7 @@ __device__ __forceinline__ void bs_const_mul(const uint32_t (&p)[4], uint32_t (&
 // once: row m + t folds into rows t .. t+3 as z^t (F0 + F1 z + F2 z^2 + F3 z^3).
 template <class LDS>
 __device__ __forceinline__ void build_columns(const LDS &S, uint32_t (&cols)[NCL][4][NG]) {
-    const int lane = threadIdx.x;
+    const int lane = lane_here();
 #pragma unroll
     for (int s = 0; s < NCL; ++s) {
         const int c = col_of(s, lane);
@@ -978,7 +988,7 @@ __device__ __forceinline__ void scale_planes(uint32_t sc, const uint64_t (&x)[4]
 // with the free variables at zero is unique, so x equals the reference's.
 template <class Sync = BlockSync, class LDS>
 __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = false) {
-    const int lane = threadIdx.x;
+    const int lane = lane_here();
     const uint8_t *rv = S.vr + R_OFF;
     // y' = y - A r: every A column times its r nibble, XOR-reduced over the
     // lanes by DPP (an LDS atomic at one address per word would become a
@@ -1193,7 +1203,7 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
 }
 
 __device__ __forceinline__ void wave_scol(Shared &S) {
-    for (int c = threadIdx.x; c < V; c += 64) {
+    for (int c = lane_here(); c < V; c += 64) {
         uint64_t v = 0;
         for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(S.vr + j * V_BYTES, c) << (4 * j);
         S.scol[c] = v;
@@ -1203,7 +1213,7 @@ __device__ __forceinline__ void wave_scol(Shared &S) {
 // s_i = (V_i + O x_i || x_i) and the salt (sign.cu:147-191, 472-477)
 template <class LDS>
 __device__ __forceinline__ void wave_write_sig(const LDS &S, const SignItem &it) {
-    const int lane = threadIdx.x;
+    const int lane = lane_here();
     const uint8_t *Ob = it.esk + SK;
     for (int b = lane; b < SIG_S_BYTES; b += 64) {
         uint32_t out = 0;
@@ -1230,7 +1240,7 @@ __device__ __forceinline__ void wave_write_sig(const LDS &S, const SignItem &it)
 // SHAKE256(digest || salt || seed || c) into S.vr.
 template <class Sync>
 __device__ __forceinline__ void derive_vr(Shared &S, const SignItem &it, int c) {
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_here() & 63;
     for (int b = lane; b < DIGEST + SK; b += 64) S.hbuf[b] = it.scratch[SCR_DIGEST + b];
     for (int b = lane; b < SK; b += 64) S.hbuf[DIGEST + SK + b] = it.esk[b];
     if (lane == 0) S.hbuf[DIGEST + 2 * SK] = (uint8_t)c;
