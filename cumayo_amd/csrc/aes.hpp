@@ -98,4 +98,71 @@ __device__ __forceinline__ void ctr_block(const uint32_t *rk, const Replicated &
     o[3] = last_col(tb, off, s3, s0, s1, s2, rk[43]);
 }
 
+// ---- counter blocks of one key, a wavefront at a time ----------------------
+// The counter is 0^64 || BE64(blk) with blk < 2^16, so the state words after
+// AddRoundKey are s0, s1, s2 = key words and s3 = rk[3] ^ blk. Across the 64
+// blocks base + lane of one wavefront (base a multiple of 64), bytes 1..3 of
+// s3 are uniform: after round 1 only column 0 varies (through T3[s3.b0]),
+// column 1 changes with blk's byte 1, columns 2 and 3 are fixed per key; in
+// round 2 each column takes one lookup of column 0 and three uniform ones.
+// The uniform lookups run on the scalar unit from the constant T0 (scalar
+// loads and rotations), once per key or per wavefront step; a block then
+// costs 1 + 4 lookups for rounds 1 and 2 instead of 4 + 16.
+__device__ __forceinline__ uint32_t ror_u(uint32_t x, int s) { return s ? (x >> s) | (x << (32 - s)) : x; }
+
+// Tt[byte B of x] for a wave-uniform x, from the constant table T0
+template <int T, int B>
+__device__ __forceinline__ uint32_t ut(const uint32_t *T0, uint32_t x) {
+    return ror_u(T0[(x >> (8 * B)) & 0xFFu], 8 * T);
+}
+
+struct CtrKey {
+    const uint32_t *rk;      // 44 round keys (SGPRs)
+    uint32_t a0, a1, t2, t3; // round-1 parts fixed per key
+    __device__ __forceinline__ CtrKey(const uint32_t *rk_, const uint32_t *T0) : rk(rk_) {
+        const uint32_t s0 = rk[0], s1 = rk[1], s2 = rk[2], s3 = rk[3];  // bytes 2, 3 of s3 only
+        a0 = ut<0, 3>(T0, s0) ^ ut<1, 2>(T0, s1) ^ ut<2, 1>(T0, s2) ^ rk[4];
+        a1 = ut<0, 3>(T0, s1) ^ ut<1, 2>(T0, s2) ^ ut<3, 0>(T0, s0) ^ rk[5];
+        t2 = ut<0, 3>(T0, s2) ^ ut<1, 2>(T0, s3) ^ ut<2, 1>(T0, s0) ^ ut<3, 0>(T0, s1) ^ rk[6];
+        t3 = ut<0, 3>(T0, s3) ^ ut<1, 2>(T0, s0) ^ ut<2, 1>(T0, s1) ^ ut<3, 0>(T0, s2) ^ rk[7];
+    }
+};
+
+// round-2 parts of one wavefront step (blocks base .. base + 63)
+struct CtrStep {
+    uint32_t c[4];
+    __device__ __forceinline__ CtrStep(const CtrKey &k, const uint32_t *T0, uint32_t base) {
+        const uint32_t t1 = k.a1 ^ ut<2, 1>(T0, k.rk[3] ^ base);
+        c[0] = ut<1, 2>(T0, t1) ^ ut<2, 1>(T0, k.t2) ^ ut<3, 0>(T0, k.t3) ^ k.rk[8];
+        c[1] = ut<0, 3>(T0, t1) ^ ut<1, 2>(T0, k.t2) ^ ut<2, 1>(T0, k.t3) ^ k.rk[9];
+        c[2] = ut<0, 3>(T0, k.t2) ^ ut<1, 2>(T0, k.t3) ^ ut<3, 0>(T0, t1) ^ k.rk[10];
+        c[3] = ut<0, 3>(T0, k.t3) ^ ut<2, 1>(T0, t1) ^ ut<3, 0>(T0, k.t2) ^ k.rk[11];
+    }
+};
+
+// The block base + lane of the step: rounds 1 and 2 from the uniform parts,
+// then rounds 3..10 as ctr_block.
+__device__ __forceinline__ void ctr_block_step(const CtrKey &k, const CtrStep &st, const Replicated &tb,
+                                               uint32_t blk, uint32_t o[4]) {
+    const uint32_t off = copy_offset();
+    const uint32_t *rk = k.rk;
+    const uint32_t t0 = k.a0 ^ ror(rt<0>(tb, rk[3] ^ blk, off), 24);  // T3[s3.b0]
+    uint32_t s0 = st.c[0] ^ rt<3>(tb, t0, off);                        // T0[t0.b3]
+    uint32_t s1 = st.c[1] ^ ror(rt<0>(tb, t0, off), 24);               // T3[t0.b0]
+    uint32_t s2 = st.c[2] ^ ror(rt<1>(tb, t0, off), 16);               // T2[t0.b1]
+    uint32_t s3 = st.c[3] ^ ror(rt<2>(tb, t0, off), 8);                // T1[t0.b2]
+#pragma unroll
+    for (int r = 3; r < 10; ++r) {
+        const uint32_t t0_ = round_col(tb, off, s0, s1, s2, s3, rk[4 * r]);
+        const uint32_t t1_ = round_col(tb, off, s1, s2, s3, s0, rk[4 * r + 1]);
+        const uint32_t t2_ = round_col(tb, off, s2, s3, s0, s1, rk[4 * r + 2]);
+        const uint32_t t3_ = round_col(tb, off, s3, s0, s1, s2, rk[4 * r + 3]);
+        s0 = t0_; s1 = t1_; s2 = t2_; s3 = t3_;
+    }
+    o[0] = last_col(tb, off, s0, s1, s2, s3, rk[40]);
+    o[1] = last_col(tb, off, s1, s2, s3, s0, rk[41]);
+    o[2] = last_col(tb, off, s2, s3, s0, s1, rk[42]);
+    o[3] = last_col(tb, off, s3, s0, s1, s2, rk[43]);
+}
+
 }  // namespace aes

@@ -420,9 +420,14 @@ __global__ void __launch_bounds__(AES_WG, 8) k_aes_expand(uint8_t *pool, size_t 
             for (int i = 0; i < 44; ++i) rks[i] = __builtin_amdgcn_readfirstlane(rk[row][i]);
             uint8_t *out = pool + (k0 + row) * pitch + out_off;
             const bool aligned = (((uintptr_t)out) & 3) == 0;
-            for (int blk = sub * 64 + lane; blk < AES_BLOCKS; blk += 64 * kw) {
+            static_assert(AES_BLOCKS <= 65536, "counter blocks below 2^16");
+            const aes::CtrKey key(rks, c_aes_T0);
+            for (int base = sub * 64; base < AES_BLOCKS; base += 64 * kw) {
+                const aes::CtrStep step(key, c_aes_T0, (uint32_t)base);
+                const int blk = base + lane;
+                if (blk >= AES_BLOCKS) break;
                 uint32_t o[4];
-                aes::ctr_block(rks, tb, (uint64_t)blk, o);
+                aes::ctr_block_step(key, step, tb, (uint32_t)blk, o);
                 uint8_t *p = out + 16 * (size_t)blk;
                 int nb = P_BYTES - 16 * blk;
                 if (nb >= 16 && aligned) {
