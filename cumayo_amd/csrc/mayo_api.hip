@@ -146,11 +146,19 @@ void copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t wid
 }
 
 // crypto_sign_signWithesk signs without L (1) or expands L and signs with it
-// as the reference does (0).
+// as the reference does (0). Without L costs K (P1 entries + V O) MACs per
+// signature against V^2 O for L, which pays for small K/O: over distinct
+// keys (64-byte messages) MAYO_2 signs at 6.8 M/s without L against 4.8 M/s
+// with it; MAYO_3 at 0.98 against 0.94 M/s; MAYO_1 (2.60 against 2.72 M/s)
+// and MAYO_5 (0.29 against 0.35 M/s) are faster with L.
 #ifndef MAYO_SIGNW_LFREE
-#define MAYO_SIGNW_LFREE 1
+#define MAYO_SIGNW_LFREE (MAYO_VERSION == 2)
 #endif
 constexpr int SIGNW_LFREE = MAYO_SIGNW_LFREE;
+
+// Test hook: the first attempts of every item of the pitched signing kernels
+// report a singular system (0 = off).
+std::atomic<int> g_pitched_fail_first{0};
 
 // Pool offsets of the sign layouts (api.cu:97-104, 125-132).
 constexpr size_t SIGN_ESK_LEN = ESK_BYTES;
@@ -183,10 +191,11 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
     PitchedSign acc{pool, pitch, msg_off, sig_off, scr_off, (uint32_t)mlen};
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
+    const int ff = g_pitched_fail_first.load(std::memory_order_relaxed);
     if (lfree)
-        wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys, true><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+        wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys, true><<<(unsigned)n, 64, 0, stream>>>(acc, n, ff);
     else
-        wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+        wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n, ff);
     note_launch();
 }
 
@@ -384,6 +393,12 @@ void cumayo_get_sizes(cumayo_sizes *out) {
 }
 
 int cumayo_last_error(void) { return g_last_error.exchange(0); }
+
+int cumayo_signwithesk_lfree(void) { return SIGNW_LFREE != 0; }
+
+void cumayo_test_pitched_fail_first(int attempts) {
+    g_pitched_fail_first.store(attempts > 0 ? (attempts < MAX_ATTEMPTS ? attempts : MAX_ATTEMPTS) : 0);
+}
 
 size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_STRIDE; }
 

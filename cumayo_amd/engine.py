@@ -113,6 +113,8 @@ EXPORTS = {
     "aes128_tables_init": [],
     "cumayo_get_sizes": [ctypes.POINTER(_Sizes)],
     "cumayo_last_error": [],
+    "cumayo_signwithesk_lfree": [],
+    "cumayo_test_pitched_fail_first": [ctypes.c_int],
     "cumayo_sign_workspace_size": [_SZ],
     "cumayo_sign_table": [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _VP],
     "cumayo_verify_workspace_size": [_SZ],
@@ -142,6 +144,7 @@ EXPORTS = {
 }
 _RESTYPES = {
     "cumayo_last_error": ctypes.c_int,
+    "cumayo_signwithesk_lfree": ctypes.c_int,
     "cumayo_sign_workspace_size": _SZ,
     "cumayo_verify_workspace_size": _SZ,
     "cumayo_sign_table": ctypes.c_int,

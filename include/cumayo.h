@@ -74,8 +74,9 @@ void crypto_sign_sign(uint8_t *d_sign_mem_pool, size_t Mlen, size_t pitch, size_
 void crypto_sign_sign_DtH(uint8_t *sig, uint8_t *d_sig_mem_pool, size_t Mlen, size_t pitch,
                           size_t batch_size, hipStream_t stream CUMAYO_DEFAULT(nullptr));
 /* api.cuh:31-35 (api.cu:116-144): compact sk in, expanded then signed;
- * rows A128(esk) || A128(msg) || A128(sig) || scratch. Signs without L: after
- * the call the esk region holds sk || O || P1 || P2 (the reference leaves
+ * rows A128(esk) || A128(msg) || A128(sig) || scratch. Where
+ * cumayo_signwithesk_lfree() is 1 (MAYO_2) it signs without L: after the
+ * call the esk region holds sk || O || P1 || P2 (the reference leaves
  * sk || O || P1 || L there); the signatures are the reference's. */
 void crypto_sign_signWithesk_HtD(uint8_t *cesk, uint8_t *cMess, size_t mlen,
                                  uint8_t *d_sign_mem_pool, size_t pitch, size_t batch_size,
@@ -124,6 +125,15 @@ void cumayo_get_sizes(cumayo_sizes *out);
 /* Last HIP error recorded by any entry point of this library (0 = none);
  * reading clears it. */
 int cumayo_last_error(void);
+/* 1 when crypto_sign_signWithesk signs without L in this library (the
+ * parameter sets where that is faster; the pool then holds P2 in L's place),
+ * 0 when it expands and uses L as the reference does. */
+int cumayo_signwithesk_lfree(void);
+/* Test hook (no counterpart in the reference): with attempts > 0, the first
+ * `attempts` attempts of every item of the drop-in signing kernels
+ * (crypto_sign_sign, crypto_sign_signWithesk) report a singular system, as
+ * a rank failure would; 0 turns it off. */
+void cumayo_test_pitched_fail_first(int attempts);
 
 /* Batched signing over a table of expanded secret keys (reference esk
  * format, rows esk_stride bytes apart): item i signs msg_len[i] bytes at

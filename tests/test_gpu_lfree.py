@@ -1,16 +1,19 @@
-"""crypto_sign_signWithesk signs without L: the pool rows keep P2 where the
-reference's expansion writes L = (P1 + P1^T) O + P2 (expandsk.cu:37-81), and
-the signing kernel forms M_j = V_j^T P2 + (W_j)^T O from W_j = (P1 + P1^T)
-V_j instead of V_j^T L (sign.cu:256-330).
+"""crypto_sign_signWithesk for MAYO_2 signs without L: the pool rows keep P2
+where the reference's expansion writes L = (P1 + P1^T) O + P2
+(expandsk.cu:37-81), and the signing kernel forms M_j = V_j^T P2 + (W_j)^T O
+from W_j = (P1 + P1^T) V_j instead of V_j^T L (sign.cu:256-330). The other
+sets expand L and sign with it (faster there).
 
-* every parameter set: distinct compact keys, one message each, the L-free
-  signatures equal, item by item, those of crypto_sign_expandsk followed by
-  crypto_sign_sign (which expand and use L) over the same keys and
-  messages, and a sample equals the oracle's;
-* the two natural second attempts of the MAYO_1 retry workload (a rank
-  failure on attempt 0, then a new V from the attempt counter) through the
-  L-free path, against the oracle;
-* the pool after the call holds P1 || P2 in the esk's P1 || L place.
+* every parameter set: distinct compact keys, one message each, the
+  signWithesk signatures equal, item by item, those of crypto_sign_expandsk
+  followed by crypto_sign_sign (which expand and use L) over the same keys
+  and messages, and a sample equals the oracle's;
+* forced rank failures of the first two attempts (test hook) through the
+  L-free kernel and the L kernel: third-attempt signatures equal to each
+  other and to the oracle's under the same forced failures;
+* the two natural second attempts of the MAYO_1 retry workload through the
+  drop-in path, against the oracle;
+* the L-free pool after the call holds P1 || P2 in the esk's P1 || L place.
 """
 import numpy as np
 import pytest
@@ -50,6 +53,29 @@ def test_lfree_equals_L_path(engine_factory, v):
     assert int((res != 0).sum()) == 0
 
 
+def test_lfree_forced_retries(engine_factory):
+    v, n, ff = 2, 300, 2
+    eng = engine_factory(v)
+    assert eng.lib.cumayo_signwithesk_lfree() == 1
+    _, sk = eng.keypair(n, rand_index=11)
+    msgs = messages(n, 33, "ff")
+    print(f"[lfree] forced retries MAYO_{v} n={n} fail_first={ff}", flush=True)
+    eng.lib.cumayo_test_pitched_fail_first(ff)
+    try:
+        sig = eng.sign_with_sk(sk, msgs)
+        esk = eng.expandsk(sk)
+        sig_L = eng.sign(esk, msgs)
+    finally:
+        eng.lib.cumayo_test_pitched_fail_first(0)
+    assert np.array_equal(sig, sig_L)
+    for i in list(range(0, n, 37)) + [n - 1]:
+        osig, oatt = O.sign(v, esk[i].tobytes(), msgs[i].tobytes(), fail_first=ff)
+        assert oatt == ff + 1
+        assert sig[i].tobytes() == osig, f"item {i}"
+    # the hook is off again: first-attempt signatures
+    assert eng.sign_with_sk(sk[:1], msgs[:1])[0].tobytes() == O.sign(v, esk[0].tobytes(), msgs[0].tobytes())[0]
+
+
 def test_lfree_natural_retries_mayo1(engine_factory):
     """Items 2768 and 6420 of the bench generator's "retry-test" workload
     (16 keys, key j mod 16, lengths in [32, 1024]) need a second attempt;
@@ -76,6 +102,7 @@ def test_lfree_pool_keeps_P2(engine_factory):
     from cumayo_amd.engine import a128
     v, n, mlen = 2, 4, 40
     eng = engine_factory(v)
+    assert eng.lib.cumayo_signwithesk_lfree() == 1
     z = eng.sizes
     pk, sk = eng.keypair(n, rand_index=9)
     msgs = messages(n, mlen, "pool")
