@@ -476,13 +476,14 @@ hipStream_t aux_stream(hipStream_t caller) {
 // other half's tails, and each half's retry pass (over the half's own
 // failure list) runs beside the other half's kernels.
 template <class KL, class PK, class SK>
-hipError_t launch_sign_stage2(PK prep_k, SK solve_k, const TableSign &acc, size_t n, int ordered, int fail_first,
-                              int parts, hipStream_t stream) {
+hipError_t launch_sign_stage2(PK prep_k, SK solve_k, int solve_w, const TableSign &acc, size_t n, int ordered,
+                              int fail_first, int parts, hipStream_t stream) {
     const unsigned retry_threads = 64 * wv::RNW;
     // prep, solve and retry over the positions [base, base + m) on s
     auto part = [&](size_t base, size_t m, hipStream_t s) {
         hipLaunchKernelGGL(prep_k, dim3((unsigned)m), dim3(64), 0, s, acc, n, ordered, base);
-        hipLaunchKernelGGL(solve_k, dim3((unsigned)m), dim3(64), 0, s, acc, n, fail_first, ordered, base);
+        hipLaunchKernelGGL(solve_k, dim3((unsigned)((m + solve_w - 1) / solve_w)), dim3(64 * solve_w), 0, s, acc, n,
+                           fail_first, ordered, base, base + m);
         hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3((unsigned)(m < 256 ? m : 256)), dim3(retry_threads), 0, s,
                            acc, n, fail_first, base);
     };
@@ -575,7 +576,8 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
                         found = true;
                         e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL, decltype(P)::value>,
                                                    &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
-                                                   acc, n, ordered, fail_first, parts, stream);
+                                                   wv::solve_sigs(decltype(Q)::value), acc, n, ordered, fail_first,
+                                                   parts, stream);
                     });
                 });
                 if (!found) {
@@ -584,7 +586,7 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
                 }
             } else {
                 e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL>, &wv::k_sign_solve<TableSign, KL::PLANAR>,
-                                           acc, n, ordered, fail_first, parts, stream);
+                                           1, acc, n, ordered, fail_first, parts, stream);
             }
         }
         if (e == hipSuccess) e = hipGetLastError();

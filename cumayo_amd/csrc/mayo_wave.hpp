@@ -60,7 +60,15 @@ constexpr int SOLVE_OCC = M <= 80 ? 8 : 3;
 constexpr int qf_code(int rt, int occ) { return 10 * rt + occ; }
 inline constexpr int PREP_OCCS[4] = {qf_code(QF_RT, QF_OCC), qf_code(1, K >= 12 ? 3 : QF_OCC + 1), qf_code(3, 2),
                                      qf_code(QF_RT, QF_OCC - 1)};
-inline constexpr int SOLVE_OCCS[3] = {SOLVE_OCC, M <= 80 ? 6 : 2, 4};
+// Solve kernel codes: 100 (signatures per workgroup - 1) + 10 schedule +
+// waves per SIMD. Schedule 0 searches the pivot row on the scalar unit from
+// the broadcast pivot column, 1 has every lane search its own column on the
+// vector unit (a VALU for SALU trade); with two signatures per workgroup the
+// two waves keep their own LDS and never wait for each other.
+inline constexpr int SOLVE_OCCS[5] = {SOLVE_OCC, M <= 80 ? 6 : 2, 4, 10 + SOLVE_OCC, 100 + SOLVE_OCC};
+constexpr int solve_occ(int code) { return code % 10; }
+constexpr int solve_schedule(int code) { return (code / 10) % 10; }
+constexpr int solve_sigs(int code) { return code / 100 + 1; }
 inline constexpr int VERIFY_OCCS[4] = {qf_code(QF_RT, QF_OCC), qf_code(QF_RT, QF_OCC - 1),
                                        qf_code(1, K >= 12 ? 3 : QF_OCC + 1), qf_code(3, 2)};
 
@@ -93,12 +101,13 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | (uint64_t)uni((uint32_t)x);
 }
 
-// threadIdx.x behind an empty asm statement. The phases of the whole-loop
-// signing kernels derive per-lane constants from it; taken this way inside
-// each phase they are recomputed there, instead of being hoisted out of the
-// attempt loop and held (spilled) across the quadratic form.
+// The lane within the wavefront (threadIdx.x mod 64) behind an empty asm
+// statement. The phases of the whole-loop signing kernels derive per-lane
+// constants from it; taken this way inside each phase they are recomputed
+// there, instead of being hoisted out of the attempt loop and held (spilled)
+// across the quadratic form.
 __device__ __forceinline__ int lane_here() {
-    int l = (int)threadIdx.x;
+    int l = (int)(threadIdx.x & 63u);
     asm volatile("" : "+v"(l));
     return l;
 }
@@ -986,7 +995,7 @@ __device__ __forceinline__ void scale_planes(uint32_t sc, const uint64_t (&x)[4]
 // Pivot columns are the greedy ones in column order, as in the reference's
 // echelon form (sign.cu:10-145), and z on them is y'[pr] / p: the solution
 // with the free variables at zero is unique, so x equals the reference's.
-template <class Sync = BlockSync, class LDS>
+template <class Sync = BlockSync, bool VS = false, class LDS>
 __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = false) {
     const int lane = lane_here();
     const uint8_t *rv = S.vr + R_OFF;
@@ -1057,12 +1066,24 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                     fw[d][h] = ((uint64_t)hi << 32) | lo;
                 }
             int pr = -1;
+            if constexpr (VS) {
+                // every lane searches its own column of the set; the pivot
+                // lane's answer is read back
 #pragma unroll
-            for (int h = NH - 1; h >= 0; --h) {
-                const uint64_t nz = (fw[0][h] | fw[1][h] | fw[2][h] | fw[3][h]) & ~used[h];
-                pr = nz ? 64 * h + (int)__builtin_ctzll(nz) : pr;
+                for (int g = NG - 1; g >= 0; --g) {
+                    const uint32_t ug = (g & 1) ? (uint32_t)(used[g >> 1] >> 32) : (uint32_t)used[g >> 1];
+                    const uint32_t nz = (cols[ps][0][g] | cols[ps][1][g] | cols[ps][2][g] | cols[ps][3][g]) & ~ug;
+                    pr = nz ? 32 * g + (int)__builtin_ctz(nz) : pr;
+                }
+                pr = __builtin_amdgcn_readlane(pr, pl);
+            } else {
+#pragma unroll
+                for (int h = NH - 1; h >= 0; --h) {
+                    const uint64_t nz = (fw[0][h] | fw[1][h] | fw[2][h] | fw[3][h]) & ~used[h];
+                    pr = nz ? 64 * h + (int)__builtin_ctzll(nz) : pr;
+                }
+                pr = __builtin_amdgcn_readfirstlane(pr);
             }
-            pr = __builtin_amdgcn_readfirstlane(pr);
             if (pr >= 0) {
                 const int pg = pr >> 5;
                 const uint32_t pb = (uint32_t)pr & 31u;
@@ -1420,29 +1441,38 @@ __global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, 
 
 // Batch form, attempt 0, part 2: columns of [A | y - A r], Gauss-Jordan,
 // signature. Few registers, so many waves per SIMD hide the pivot chain.
-// PL: the M vectors in the workspace are planar (prepared keys).
-template <class Acc, bool PL, int OCC = SOLVE_OCC>
-__global__ void __launch_bounds__(64, OCC) k_sign_solve(Acc acc, size_t n, int fail_first = 0, int ordered = 0,
-                                                       size_t base = 0) {
-    __shared__ SolveShared S;
-    const int lane = threadIdx.x;
-    const size_t item = item_of(acc, ordered, base);
+// PL: the M vectors in the workspace are planar (prepared keys). CODE: a
+// solve variant (SOLVE_OCCS): waves per SIMD, pivot search schedule, and
+// signatures per workgroup (one wave each, positions base + blockIdx.x W +
+// wave below end).
+template <class Acc, bool PL, int CODE = SOLVE_OCC>
+__global__ void __launch_bounds__(64 * solve_sigs(CODE), solve_occ(CODE))
+    k_sign_solve(Acc acc, size_t n, int fail_first = 0, int ordered = 0, size_t base = 0, size_t end = ~(size_t)0) {
+    constexpr int W = solve_sigs(CODE);
+    using Sync = std::conditional_t<(W > 1), WaveSync, BlockSync>;
+    __shared__ SolveShared Sw[W];
+    const int wave = W > 1 ? (int)(threadIdx.x >> 6) : 0;
+    SolveShared &S = Sw[wave];
+    const int lane = threadIdx.x & 63;
+    const size_t k = base + (size_t)blockIdx.x * W + wave;
+    if (W > 1 && k >= end) return;  // whole waves only; no workgroup barrier follows
+    const size_t item = ordered ? acc.order(k) : k;
     SignItem it = acc(item);
     for (int b = lane; b < VR_BYTES; b += 64) S.vr[b] = it.scratch[SCR_VR + b];
     planes_of_M<PL>((const uint32_t *)(it.scratch + SCR_MV), S.Mb);
     const uint32_t *yv = (const uint32_t *)(it.scratch + SCR_Y);
     for (int e = lane; e < MW; e += 64) S.y[e] = yv[e];
     for (int e = lane; e < 256; e += 64) S.mul[e] = (uint8_t)gf16::mul(e >> 4, e & 15);
-    __syncthreads();
+    Sync::sync();
     uint32_t cols[NCL][4][NG];
     build_columns(S, cols);
-    const int ok = solve(S, cols, fail_first > 0);
+    const int ok = solve<Sync, solve_schedule(CODE) == 1>(S, cols, fail_first > 0);
     if (!ok) {
         // pending: listed for the retry pass
         if (lane == 0) {
             it.scratch[SCR_ATTEMPTS] = ATTEMPT_PENDING;
-            const uint32_t k = atomicAdd((uint32_t *)(acc.scratch_of(base) + SCR_LIST_COUNT), 1u);
-            *(uint32_t *)(acc.scratch_of(base + k) + SCR_LIST) = (uint32_t)item;
+            const uint32_t kk = atomicAdd((uint32_t *)(acc.scratch_of(base) + SCR_LIST_COUNT), 1u);
+            *(uint32_t *)(acc.scratch_of(base + kk) + SCR_LIST) = (uint32_t)item;
         }
         return;
     }

@@ -354,7 +354,9 @@ int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
  * only). Every configuration gives the same bytes. */
 typedef struct {
     int32_t sign_prep;    /* quadratic form + M + y kernel (10 x rows per lane + waves per SIMD) */
-    int32_t sign_solve;   /* columns of A + elimination kernel (waves per SIMD) */
+    int32_t sign_solve;   /* columns of A + elimination kernel: 100 (signatures per
+                             workgroup - 1) + 10 pivot-search schedule (0 scalar
+                             unit, 1 vector unit) + waves per SIMD */
     int32_t sign_order;   /* 0 batch order, 1 XCD-aware key order */
     int32_t verify;       /* verification kernel (10 x rows per lane + waves per SIMD) */
     int32_t verify_order;
