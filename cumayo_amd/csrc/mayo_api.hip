@@ -192,6 +192,16 @@ void launch_sign_pitched(uint8_t *pool, size_t pitch, size_t n, size_t msg_off, 
     k_sign_hash<PitchedSign><<<blocks64(n), 64, 0, stream>>>(acc, n);
     note_launch();
     const int ff = g_pitched_fail_first.load(std::memory_order_relaxed);
+    if constexpr (wv::REF_WORD_ALIGNED) {
+        if (((uintptr_t)pool | pitch) % 4 == 0) {
+            if (lfree)
+                wv::k_sign_wave<PitchedSign, wv::PitchedAlignedKeys, true><<<(unsigned)n, 64, 0, stream>>>(acc, n, ff);
+            else
+                wv::k_sign_wave<PitchedSign, wv::PitchedAlignedKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n, ff);
+            note_launch();
+            return;
+        }
+    }
     if (lfree)
         wv::k_sign_wave<PitchedSign, wv::PitchedRefKeys, true><<<(unsigned)n, 64, 0, stream>>>(acc, n, ff);
     else
@@ -203,6 +213,13 @@ void launch_verify_pitched(uint8_t *pool, size_t pitch, size_t n, size_t mlen, h
     if (!n || !batch_ok(n)) return;
     const size_t msg_off = EPK_PAD, sig_off = EPK_PAD + A128(mlen), res_off = sig_off + SIG_PAD;
     PitchedVerify acc{pool, pitch, msg_off, sig_off, res_off, (uint32_t)mlen};
+    if constexpr (wv::REF_WORD_ALIGNED) {
+        if (((uintptr_t)pool | pitch) % 4 == 0) {
+            wv::k_verify_wave<PitchedVerify, wv::PitchedAlignedKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
+            note_launch();
+            return;
+        }
+    }
     wv::k_verify_wave<PitchedVerify, wv::PitchedRefKeys><<<(unsigned)n, 64, 0, stream>>>(acc, n);
     note_launch();
 }

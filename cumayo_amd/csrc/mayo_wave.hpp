@@ -223,6 +223,14 @@ struct RefKeys {
 struct PitchedRefKeys : RefKeys {
     static constexpr int AHEAD = M <= 64 ? 3 : 1;
 };
+// Reference rows whose entries all start on 4-byte boundaries: whole-word
+// entries (MB a multiple of 4, MAYO_2), every block offset a multiple of 4,
+// and a pool whose base and pitch are (checked at launch). A word of an
+// entry is then one aligned dword load, as in prepared rows.
+constexpr bool REF_WORD_ALIGNED = MB % 4 == 0 && (SK + O_BYTES) % 4 == 0 && P1_BYTES % 4 == 0 && P2_BYTES % 4 == 0;
+struct PitchedAlignedKeys : PitchedRefKeys {
+    static constexpr bool ALIGNED = true;
+};
 struct PrepKeys {
     static constexpr bool ALIGNED = true;
     static constexpr bool PLANAR = true;

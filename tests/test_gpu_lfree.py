@@ -124,3 +124,46 @@ def test_lfree_pool_keeps_P2(engine_factory):
         assert rows[i, p1_off + z.p1_bytes:p1_off + z.p_bytes].tobytes() == p2, f"row {i} P2"
         assert oesk[p1_off + z.p1_bytes:p1_off + z.p_bytes] != p2  # L differs from P2
         assert rows[i, a128(z.esk_bytes):a128(z.esk_bytes) + mlen].tobytes() == msgs[i].tobytes()
+
+
+def test_lfree_unaligned_pool(engine_factory):
+    """A pool whose pitch is not a multiple of 4 takes the unaligned-load
+    kernels (the aligned ones need 4-byte rows): MAYO_2 signWithesk and
+    verifyWithepk over such a pool give the same bytes as over an aligned one."""
+    import torch
+    from cumayo_amd.engine import a128
+    v, n, mlen = 2, 96, 45
+    eng = engine_factory(v)
+    z = eng.sizes
+    pk, sk = eng.keypair(n, rand_index=13)
+    msgs = messages(n, mlen, "odd")
+    ref = eng.sign_with_sk(sk, msgs)
+    row = a128(z.esk_bytes) + a128(mlen) + a128(z.sig_bytes) + a128(z.scratch_bytes)
+    pitch = row + 3
+    buf = torch.zeros(n * pitch + 64, dtype=torch.uint8, device=eng.device)
+    base = buf.data_ptr() + 1  # odd base as well
+    h_sk, h_msg = eng.pinned(sk.shape, sk), eng.pinned(msgs.shape, msgs)
+    sig = eng.pinned((n, z.sig_bytes))
+    st = eng.stream_handle()
+    print(f"[lfree] unaligned pool pitch={pitch}", flush=True)
+    eng.lib.crypto_sign_signWithesk_HtD(h_sk.data_ptr(), h_msg.data_ptr(), mlen, base, pitch, n, st)
+    eng.lib.crypto_sign_signWithesk(base, mlen, pitch, n, st)
+    eng.lib.crypto_sign_signWithesk_DtH(sig.data_ptr(), base, mlen, pitch, n, st)
+    eng.torch.cuda.synchronize(eng.device)
+    eng.check()
+    assert np.array_equal(sig.numpy(), ref)
+    # verification over an unaligned pool (compact pk expanded in the call)
+    epk_len = z.p_bytes + z.pk_bytes - 16
+    vrow = a128(epk_len) + a128(mlen) + a128(z.sig_bytes) + 1
+    vpitch = vrow + 1
+    vbuf = torch.zeros(n * vpitch + 64, dtype=torch.uint8, device=eng.device)
+    vbase = vbuf.data_ptr() + 2
+    h_pk = eng.pinned(pk.shape, pk)
+    h_sig = eng.pinned((n, z.sig_bytes), ref)
+    res = eng.pinned((n,))
+    eng.lib.crypto_sign_verifyWithepk_HtD(h_pk.data_ptr(), h_msg.data_ptr(), h_sig.data_ptr(), mlen, vbase, vpitch, n, st)
+    eng.lib.crypto_sign_verifyWithepk(vbase, mlen, vpitch, n, st)
+    eng.lib.crypto_sign_verifyWithepk_DtH(res.data_ptr(), mlen, vbase, vpitch, n, st)
+    eng.torch.cuda.synchronize(eng.device)
+    eng.check()
+    assert res.numpy().tolist() == [0] * n
