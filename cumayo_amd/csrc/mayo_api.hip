@@ -10,6 +10,7 @@
 #include "../../include/cumayo.h"
 #include "mayo_kernels.hpp"
 #include "mayo_wave.hpp"
+#include "mayo_shared.hpp"
 
 using namespace mayo;
 
@@ -493,12 +494,13 @@ hipStream_t aux_stream(hipStream_t caller) {
 // other half's tails, and each half's retry pass (over the half's own
 // failure list) runs beside the other half's kernels.
 template <class KL, class PK, class SK>
-hipError_t launch_sign_stage2(PK prep_k, SK solve_k, int solve_w, const TableSign &acc, size_t n, int ordered,
-                              int fail_first, int parts, hipStream_t stream) {
+hipError_t launch_sign_stage2(PK prep_k, int prep_w, SK solve_k, int solve_w, const TableSign &acc, size_t n,
+                              int ordered, int fail_first, int parts, hipStream_t stream) {
     const unsigned retry_threads = 64 * wv::RNW;
     // prep, solve and retry over the positions [base, base + m) on s
     auto part = [&](size_t base, size_t m, hipStream_t s) {
-        hipLaunchKernelGGL(prep_k, dim3((unsigned)m), dim3(64), 0, s, acc, n, ordered, base);
+        hipLaunchKernelGGL(prep_k, dim3((unsigned)((m + prep_w - 1) / prep_w)), dim3(64 * prep_w), 0, s, acc, n,
+                           ordered, base, base + m);
         hipLaunchKernelGGL(solve_k, dim3((unsigned)((m + solve_w - 1) / solve_w)), dim3(64 * solve_w), 0, s, acc, n,
                            fail_first, ordered, base, base + m);
         hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3((unsigned)(m < 256 ? m : 256)), dim3(retry_threads), 0, s,
@@ -566,14 +568,20 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
             const int fail_first = (stages >> 8) & 0xFF;
-            const int ordered = cfg.sign_order && n > 64;
+            // shared-key tables (prepared keys, split launches): items in
+            // key order, several per workgroup
+            const bool shared = std::is_same<KL, wv::PrepKeys>::value && cfg.sign_chunks != 0 && wv::sk_code(cfg.prep);
+            const int ordered = shared || (cfg.sign_order && n > 64);
             const int parts = cfg.sign_chunks > 1 && n >= CHUNK_MIN ? 2 : 1;
-            if (ordered) k_key_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
+            if (shared)
+                wv::k_key_sort<TableSign><<<1, 1024, 0, stream>>>(acc, n);
+            else if (ordered)
+                k_key_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
             if (cfg.sign_chunks == 0) {
                 // one persistent fused launch (prepared or reference tables)
                 bool found = false;
-                with_occ<wv::PREP_OCCS>(std::is_same<KL, wv::PrepKeys>::value ? cfg.prep : wv::PREP_OCCS[0],
-                                        [&](auto P) {
+                const bool own = std::is_same<KL, wv::PrepKeys>::value && !wv::sk_code(cfg.prep);
+                with_occ<wv::PREP_OCCS>(own ? cfg.prep : wv::PREP_OCCS[0], [&](auto P) {
                     constexpr int code = decltype(P)::value;
                     found = true;
                     const size_t resident = (size_t)device_simds() * (code % 10);
@@ -588,22 +596,36 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             } else if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
                 // prepared keys: the variants the launch autotuner chooses among
                 bool found = false;
-                with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto P) {
-                    with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto Q) {
-                        found = true;
-                        e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL, decltype(P)::value>,
-                                                   &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
-                                                   wv::solve_sigs(decltype(Q)::value), acc, n, ordered, fail_first,
-                                                   parts, stream);
+                if (shared) {
+                    with_occ<wv::SK_CODES>(cfg.prep, [&](auto P) {
+                        with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto Q) {
+                            found = true;
+                            e = launch_sign_stage2<KL>(&wv::k_sign_prep_shared<TableSign, decltype(P)::value>,
+                                                       wv::sk_w(decltype(P)::value),
+                                                       &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
+                                                       wv::solve_sigs(decltype(Q)::value), acc, n, ordered,
+                                                       fail_first, parts, stream);
+                        });
                     });
-                });
+                } else {
+                    with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto P) {
+                        with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto Q) {
+                            found = true;
+                            e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL, decltype(P)::value>, 1,
+                                                       &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
+                                                       wv::solve_sigs(decltype(Q)::value), acc, n, ordered,
+                                                       fail_first, parts, stream);
+                        });
+                    });
+                }
                 if (!found) {
                     note(hipErrorInvalidValue);
                     return (int)hipErrorInvalidValue;
                 }
             } else {
-                e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL>, &wv::k_sign_solve<TableSign, KL::PLANAR>,
-                                           1, acc, n, ordered, fail_first, parts, stream);
+                e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL>, 1,
+                                           &wv::k_sign_solve<TableSign, KL::PLANAR>, 1, acc, n, ordered, fail_first,
+                                           parts, stream);
             }
         }
         if (e == hipSuccess) e = hipGetLastError();
@@ -628,13 +650,22 @@ int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int 
         if (stages & 4) {
             k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
-            const int ordered = cfg.verify_order && n > 64;
-            if (ordered) k_key_order<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
+            const bool shared = std::is_same<KL, wv::PrepKeys>::value && wv::sk_code(cfg.verify);
+            const int ordered = !shared && cfg.verify_order && n > 64;
+            if (shared)
+                wv::k_key_sort<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
+            else if (ordered)
+                k_key_order<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
             if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
-                if (!with_occ<wv::VERIFY_OCCS>(cfg.verify, [&](auto O) {
-                        wv::k_verify_wave<TableVerify, KL, decltype(O)::value>
-                            <<<(unsigned)n, 64, 0, stream>>>(acc, n, ordered);
-                    })) {
+                const bool found = shared ? with_occ<wv::SK_CODES>(cfg.verify, [&](auto O) {
+                    constexpr int W = wv::sk_w(decltype(O)::value);
+                    wv::k_verify_shared<TableVerify, decltype(O)::value>
+                        <<<(unsigned)((n + W - 1) / W), 64 * W, 0, stream>>>(acc, n);
+                }) : with_occ<wv::VERIFY_OCCS>(cfg.verify, [&](auto O) {
+                    wv::k_verify_wave<TableVerify, KL, decltype(O)::value>
+                        <<<(unsigned)n, 64, 0, stream>>>(acc, n, ordered);
+                });
+                if (!found) {
                     note(hipErrorInvalidValue);
                     return (int)hipErrorInvalidValue;
                 }
@@ -931,8 +962,8 @@ void cumayo_get_launch_config(cumayo_launch_config *out) {
 }
 
 int cumayo_set_launch_config(const cumayo_launch_config *cfg) {
-    if (!cfg || !listed(wv::PREP_OCCS, cfg->sign_prep) || !listed(wv::SOLVE_OCCS, cfg->sign_solve) ||
-        !listed(wv::VERIFY_OCCS, cfg->verify) || (unsigned)cfg->sign_order > 1u ||
+    if (!cfg || !listed(wv::PREP_VARIANTS, cfg->sign_prep) || !listed(wv::SOLVE_OCCS, cfg->sign_solve) ||
+        !listed(wv::VERIFY_VARIANTS, cfg->verify) || (unsigned)cfg->sign_order > 1u ||
         (unsigned)cfg->verify_order > 1u || cfg->sign_chunks < 0 || cfg->sign_chunks > 2)
         return (int)hipErrorInvalidValue;
     g_prep.store(cfg->sign_prep);
@@ -951,9 +982,9 @@ int cumayo_launch_variants(int field, int32_t *out, int max) {
     static const int chunks[3] = {1, 2, 0};
     auto use = [&](const auto &arr) { v = arr, cnt = (int)(sizeof(arr) / sizeof(arr[0])); };
     switch (field) {
-        case CUMAYO_FIELD_SIGN_PREP: use(wv::PREP_OCCS); break;
+        case CUMAYO_FIELD_SIGN_PREP: use(wv::PREP_VARIANTS); break;
         case CUMAYO_FIELD_SIGN_SOLVE: use(wv::SOLVE_OCCS); break;
-        case CUMAYO_FIELD_VERIFY: use(wv::VERIFY_OCCS); break;
+        case CUMAYO_FIELD_VERIFY: use(wv::VERIFY_VARIANTS); break;
         case CUMAYO_FIELD_ORDER: use(orders); break;
         case CUMAYO_FIELD_SIGN_CHUNKS: use(chunks); break;
         default: return -1;
@@ -993,10 +1024,13 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
         const int max_chunks = n >= CHUNK_MIN ? 2 : 1;
         for (int chunks = 0; chunks <= max_chunks; ++chunks)
             for (int order = 0; order < 2; ++order)
-                for (int p : wv::PREP_OCCS)
+                for (int p : wv::PREP_VARIANTS)
                     for (int so : wv::SOLVE_OCCS) {
                         if (err) break;
                         if (chunks == 0 && so != wv::SOLVE_OCCS[0]) continue;  // the fused kernel has no solve variant
+                        // the shared-key variants run in their own key order
+                        // and are not a fused-kernel option
+                        if (wv::sk_code(p) && (order == 1 || chunks == 0)) continue;
                         LaunchCfg c = cfg;
                         c.prep = p, c.solve = so, c.sign_order = order, c.sign_chunks = chunks;
                         const double t = time_min_ms(
@@ -1017,8 +1051,9 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
         double best = 1e30;
         LaunchCfg bc = cfg;
         for (int order = 0; order < 2; ++order)
-            for (int vo : wv::VERIFY_OCCS) {
+            for (int vo : wv::VERIFY_VARIANTS) {
                 if (err) break;
+                if (wv::sk_code(vo) && order == 1) continue;  // own key order
                 LaunchCfg c = cfg;
                 c.verify = vo, c.verify_order = order;
                 const double t = time_min_ms(

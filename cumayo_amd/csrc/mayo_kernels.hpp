@@ -98,6 +98,7 @@ struct TableSign {
     __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)SIGN_SCRATCH_STRIDE; }
     __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
     __device__ uint32_t key(size_t i) const { return (uint32_t)key_idx[i]; }
+    __device__ const uint8_t *key_row(uint32_t k) const { return esk_table + (size_t)k * esk_stride; }
     __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + SCR_ORDER) = i; }
     __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + SCR_ORDER); }
 };
@@ -140,6 +141,7 @@ struct TableVerify {
     __device__ uint8_t *scratch_of(size_t i) const { return scratch + i * (size_t)VERIFY_SCRATCH_STRIDE; }
     __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
     __device__ uint32_t key(size_t i) const { return (uint32_t)key_idx[i]; }
+    __device__ const uint8_t *key_row(uint32_t k) const { return epk_table + (size_t)k * epk_stride; }
     __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + VSCR_ORDER) = i; }
     __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + VSCR_ORDER); }
 };

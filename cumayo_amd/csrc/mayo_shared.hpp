@@ -1,0 +1,612 @@
+// Shared-key quadratic forms: a workgroup of W wavefronts signs (or
+// verifies) up to W items under one key, one wavefront per item.
+//
+// The quadratic form's column loop needs, for every entry P[r][c] of the key
+// and every scalar S_j[c] of an item, the product S_j[c] P[r][c]. A product
+// depends on the key entry and on a nibble only, so the workgroup builds, in
+// LDS, the 16 multiples s P[r][c] (s = 0..15) of the entries of a few
+// columns of the current row chunk once, and every wavefront of the group
+// then takes each of its products as one LDS read at an address set by its
+// scalar (uniform across the wave, so the read is conflict-free) and one
+// XOR: 1.5 VALU instructions per entry word and scalar against 4 AND-XORs
+// in the bitsliced form, with the table built once per W items. The loop is
+// bound by LDS bandwidth (8-byte reads: about 120 bytes per clock and CU).
+//
+// Lanes of the column loop are (word pair g, row slot): a chunk holds one
+// row per slot, and each lane accumulates T_j[r] = sum_c P[r][c] S_j[c] for
+// its row and its two words, for every j, in 2K registers. Tables are
+// double-buffered: while the wavefronts read the multiples of columns
+// [c0, c0 + CG) the producing threads write those of the next CG columns,
+// and one workgroup barrier per CG columns separates the two.
+//
+// At a chunk's end the lanes store T (and, for the first NLR rows of
+// signing, which are the columns x of L, M_j[x] = T_j[x]) and the wave folds
+// the chunk's P rows into the pair sums, U_(a,j) += S_a[r] T_j[r], in a
+// transposed layout: lanes own (j, word pair) items and keep U_(a,j) for all
+// a in registers (2K per item), rows are uniform, so the masks of S_a[r] are
+// SGPR masks. U_(a,j) + U_(j,a) is the pair sum of the bitsliced kernels
+// (sign.cu:395-437, verify.cu:72-182).
+//
+// The items of a workgroup are W consecutive positions of the key order
+// (k_key_sort), so they share a key except at the boundaries between keys;
+// a workgroup whose items hold several keys runs the table loop once per
+// key, each wavefront reading in the pass for its own key.
+#pragma once
+#include "mayo_wave.hpp"
+
+namespace mayo {
+namespace wv {
+
+// ---- key order ------------------------------------------------------------------
+// Items sorted by key (counting sort over up to SORT_BINS bins: exact for key
+// ranges up to 4,096, the top 12 bits of the key above), sorted position k
+// to acc.set_order(k). One workgroup; positions within a key in any order.
+constexpr int SORT_BINS = 4096;
+template <class Acc>
+__global__ void __launch_bounds__(1024) k_key_sort(Acc acc, size_t n) {
+    constexpr int PER = SORT_BINS / 1024;
+    __shared__ uint32_t pos[SORT_BINS];
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t kmax;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) kmax = 0;
+    for (int b = t; b < SORT_BINS; b += 1024) pos[b] = 0;
+    __syncthreads();
+    uint32_t km = 0;
+    for (size_t i = t; i < n; i += 1024) km = max(km, acc.key(i));
+    atomicMax(&kmax, km);
+    __syncthreads();
+    const int bits = kmax ? 32 - __builtin_clz(kmax) : 0;
+    const int sh = bits > 12 ? bits - 12 : 0;  // key >> sh < SORT_BINS
+    for (size_t i = t; i < n; i += 1024) atomicAdd(&pos[acc.key(i) >> sh], 1u);
+    __syncthreads();
+    // exclusive scan: thread t owns bins PER t .. PER t + PER - 1
+    uint32_t v[PER], s = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        v[u] = pos[PER * t + u];
+        s += v[u];
+    }
+    uint32_t incl = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t up = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += up;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = incl - s;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        pos[PER * t + u] = before;
+        before += v[u];
+    }
+    __syncthreads();
+    for (size_t i = t; i < n; i += 1024) {
+        const uint32_t k = atomicAdd(&pos[acc.key(i) >> sh], 1u);
+        acc.set_order(k, (uint32_t)i);
+    }
+}
+
+// ---- shared-key tables ----------------------------------------------------------
+constexpr int SK_MWL = 2;                  // entry words per lane in the column loop
+static_assert(MW % SK_MWL == 0, "m-vectors of whole word pairs");
+constexpr int SK_NWP = MW / SK_MWL;        // word pairs per entry
+constexpr int SK_RPC = 64 / SK_NWP;        // rows per chunk: one per lane slot
+constexpr int SK_CSTR = SK_RPC * MW * 4;   // bytes of one column's entries in a table
+constexpr int sk_pow2ge(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+// bytes per multiple (plane s of a table holds s times each entry word)
+template <int CG>
+constexpr int sk_sstr() {
+    return sk_pow2ge(CG * SK_CSTR);
+}
+// fold items (j, word pair) and items per lane
+constexpr int SK_NFI = K * SK_NWP;
+constexpr int SK_IPL = (SK_NFI + 63) / 64;
+
+// Shared-key variant codes (in the prep and verification variant lists):
+// 1000 + 10 W + CG, W items per workgroup, CG columns per table.
+constexpr bool sk_code(int code) { return code >= 1000; }
+constexpr int sk_w(int code) { return (code % 1000) / 10; }
+constexpr int sk_cg(int code) { return code % 10; }
+
+// LDS of one wavefront
+struct SkWave {
+    uint64_t scol[N];  // scol[c] nibble j = S_j[c]
+    union {
+        uint32_t T[SK_RPC * K * MW];  // the chunk's rows: T_j of row slot at (slot K + j) MW (planar)
+        struct {
+            uint32_t U[PAIRS * MW];     // pair sums, nibble-packed
+            uint32_t yu[YG_MAX * UW];   // unreduced y
+        } y;
+    };
+};
+
+template <int W, int CG>
+struct SkGroup {
+    uint8_t tab[2][16 * sk_sstr<CG>()] __attribute__((aligned(16)));
+    SkWave wa[W];
+    int key[W];  // key index of each wavefront's item, -1 for none
+};
+
+// Quadratic form of one key for the wavefronts with `mine` set, built by the
+// whole workgroup (every wavefront calls it; it ends with a workgroup
+// barrier). Rows: with NLR > 0 (signing), the columns x of L (entries
+// L[c][x], every c) and then the rows of P1; with FULL (verification), the
+// rows of [[P1, P2], [0, P3]] over the columns c < n. Entries come from a
+// prepared key row (aligned, planar words). U[i][a][h] accumulates
+// S_a[r] T_j[r] (word 2 gp + h) of fold item i = (j, gp); the rows of L
+// store M_j[x] to Mv (global, planar, (j O + x) MW + w).
+template <int NLR, bool FULL, int W, int CG>
+__device__ __forceinline__ void quad_form_shared(const uint8_t *key, bool mine, SkGroup<W, CG> &G, SkWave &A,
+                                                 uint32_t (&U)[SK_IPL][K][SK_MWL], uint32_t *Mv) {
+    using KL = PrepKeys;
+    constexpr int EB = KL::EB;
+    constexpr int NP = FULL ? N : V;   // P rows and columns
+    constexpr int NVR = NLR + NP;      // rows of the schedule
+    constexpr int NCH = (NVR + SK_RPC - 1) / SK_RPC;
+    constexpr int SSTR = sk_sstr<CG>();
+    constexpr int NQ = SK_RPC * CG * MW;  // words of one table
+    constexpr int NT = 64 * W;
+    constexpr int QPT = (NQ + NT - 1) / NT;
+    const int tid = threadIdx.x, lane = tid & 63;
+    mine = uni((uint32_t)mine) != 0u;  // uniform per wavefront (SGPR control flow)
+    // first column of chunk ch (its first row's start)
+    auto cstart = [](int ch) {
+        const int v0 = ch * SK_RPC;
+        return v0 < NLR ? 0 : v0 - NLR;
+    };
+    auto next = [&](int &ch, int &c0) {
+        c0 += CG;
+        if (c0 >= NP) {
+            ++ch;
+            c0 = cstart(ch);
+        }
+    };
+    // producing threads: word q of the table = (column cl, row, word w)
+    int qcl[QPT], qrow[QPT], qw[QPT];
+#pragma unroll
+    for (int u = 0; u < QPT; ++u) {
+        const int q = tid + u * NT;
+        qcl[u] = q / (SK_RPC * MW);
+        qrow[u] = (q / MW) % SK_RPC;
+        qw[u] = q % MW;
+    }
+    // Entry of (row vr, column c) at byte base[c >= V] + c stride[c >= V]
+    // for cmin <= c < NP (zero elsewhere): set per producing word when the
+    // producer moves to a new chunk.
+    int fb_ch = -1;
+    int rb0[QPT], rb1[QPT], cmin[QPT];
+    auto row_bases = [&](int ch) {
+#pragma unroll
+        for (int u = 0; u < QPT; ++u) {
+            const int vr = ch * SK_RPC + qrow[u];
+            rb0[u] = rb1[u] = 0;
+            cmin[u] = NP;  // no entries
+            if (tid + u * NT >= NQ || vr >= NVR) continue;
+            if (vr < NLR) {
+                // column x = vr of L: entry (c, x) at ESK_L + (c O + x) EB
+                rb0[u] = (int)KL::ESK_L + vr * EB + 4 * qw[u];
+                cmin[u] = 0;
+            } else {
+                const int r = vr - NLR;
+                cmin[u] = r;
+                if constexpr (FULL) {
+                    if (r < V) {
+                        rb0[u] = (int)KL::EPK_P1 + (tri_entry(r, 0, V)) * EB + 4 * qw[u];
+                        rb1[u] = (int)KL::EPK_P2 + (r * O - V) * EB + 4 * qw[u];
+                    } else {
+                        rb1[u] = (int)KL::EPK_P3 + (tri_entry(r - V, 0, O) - V) * EB + 4 * qw[u];
+                    }
+                } else {
+                    rb0[u] = (int)KL::ESK_P1 + tri_entry(r, 0, V) * EB + 4 * qw[u];
+                }
+            }
+        }
+    };
+    // entry words of interval (ch, c0), zero outside the matrix
+    auto fetch = [&](int ch, int c0, uint32_t (&x)[QPT]) {
+        if (ch != fb_ch && ch < NCH) {
+            row_bases(ch);
+            fb_ch = ch;
+        }
+#pragma unroll
+        for (int u = 0; u < QPT; ++u) {
+            const int c = c0 + qcl[u];
+            const bool lrow = NLR > 0 && ch * SK_RPC + qrow[u] < NLR;
+            // P rows: consecutive columns are consecutive entries; L columns
+            // are O entries apart
+            const int off = c < V ? rb0[u] + c * (lrow ? O * EB : EB) : rb1[u] + c * EB;
+            x[u] = ch < NCH && c >= cmin[u] && c < NP ? *(const uint32_t *)(key + off) : 0u;
+        }
+    };
+    // the 15 nonzero multiples of each word into table `buf`
+    auto produce = [&](int buf, const uint32_t (&x)[QPT]) {
+#pragma unroll
+        for (int u = 0; u < QPT; ++u) {
+            const int q = tid + u * NT;
+            if (q >= NQ) continue;
+            using namespace gf16::planar;
+            const uint32_t x1 = x[u], x2 = mul_a(x1), x4 = mul_a(x2), x8 = mul_a(x4);
+            const uint32_t x3 = x1 ^ x2, x5 = x1 ^ x4, x6 = x2 ^ x4, x7 = x3 ^ x4;
+            uint32_t *d = (uint32_t *)G.tab[buf] + q;
+            constexpr int S4 = SSTR / 4;
+            d[1 * S4] = x1;
+            d[2 * S4] = x2;
+            d[3 * S4] = x3;
+            d[4 * S4] = x4;
+            d[5 * S4] = x5;
+            d[6 * S4] = x6;
+            d[7 * S4] = x7;
+            d[8 * S4] = x8;
+            d[9 * S4] = x1 ^ x8;
+            d[10 * S4] = x2 ^ x8;
+            d[11 * S4] = x3 ^ x8;
+            d[12 * S4] = x4 ^ x8;
+            d[13 * S4] = x5 ^ x8;
+            d[14 * S4] = x6 ^ x8;
+            d[15 * S4] = x7 ^ x8;
+        }
+    };
+    // column-loop lane: word pair g of row slot `slot`. The lanes past the
+    // last slot (64 - SK_RPC SK_NWP of them) read the padding after the
+    // table's last row, which keeps them off the banks of the real rows.
+    const int g = lane % SK_NWP, slot = lane / SK_NWP;
+    const bool active = slot < SK_RPC;
+    static_assert((CG - 1) * SK_CSTR + SK_CSTR + 8 * (64 - SK_RPC * SK_NWP) <= SSTR, "idle lanes stay in the plane");
+    const int vlane = slot * MW * 4 + g * 8;
+    uint32_t T[K][SK_MWL];
+#pragma unroll
+    for (int j = 0; j < K; ++j) T[j][0] = T[j][1] = 0;
+    // scalars of the interval's columns, read one interval ahead
+    uint64_t scv[CG];
+    auto scalars = [&](int c0) {
+#pragma unroll
+        for (int cl = 0; cl < CG; ++cl) scv[cl] = c0 + cl < NP ? A.scol[c0 + cl] : 0ull;
+    };
+    // the K table reads of column cl of an interval (issued together), and
+    // their sums into T
+    auto issue = [&](int buf, int cl, uint2 (&v)[K]) {
+        const uint8_t *bb = G.tab[buf] + vlane + cl * SK_CSTR;
+        const uint64_t sc = uni64(scv[cl]);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t s = (uint32_t)(sc >> (4 * j)) & 15u;
+            v[j] = *(const uint2 *)(bb + s * SSTR);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            __builtin_amdgcn_sched_group_barrier(0x0002, 1, 0);  // the address
+            __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);  // its read
+        }
+    };
+    auto absorb = [&](const uint2 (&v)[K]) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            T[j][0] ^= v[j].x;
+            T[j][1] ^= v[j].y;
+        }
+    };
+    // The fold of one row (row `row` of the chunk starting at row v0 of the
+    // schedule) into U, in two steps so that the table reads of a column can
+    // be in flight between them: fold_load issues the row's LDS reads (its
+    // scalars and the items' T words), fold_compute does the products.
+    struct FoldIn {
+        uint64_t sr;
+        uint2 tv[SK_IPL];
+    };
+    auto fold_load = [&](int v0, int row) {
+        FoldIn f;
+        f.sr = A.scol[v0 + row - NLR];
+#pragma unroll
+        for (int i = 0; i < SK_IPL; ++i) {
+            const int it = lane + 64 * i;
+            const int j = it / SK_NWP, gp = it % SK_NWP;
+            f.tv[i] = it < SK_NFI ? *(const uint2 *)&A.T[(row * K + j) * MW + 2 * gp] : uint2{0u, 0u};
+        }
+        return f;
+    };
+    auto fold_compute = [&](const FoldIn &f) {
+        const uint64_t sr = uni64(f.sr);
+        const uint32_t sh[2] = {(uint32_t)sr, (uint32_t)(sr >> 32)};
+#pragma unroll
+        for (int i = 0; i < SK_IPL; ++i) {
+            if (lane + 64 * i >= SK_NFI) continue;
+            const gf16::planar::Multiples m0(f.tv[i].x), m1(f.tv[i].y);
+#pragma unroll
+            for (int a = 0; a < K; ++a) {
+                const int bit = 4 * a;
+                const uint32_t mk0 = gf16::bitmask(sh[bit >> 5], bit & 31);
+                const uint32_t mk1 = gf16::bitmask(sh[(bit + 1) >> 5], (bit + 1) & 31);
+                const uint32_t mk2 = gf16::bitmask(sh[(bit + 2) >> 5], (bit + 2) & 31);
+                const uint32_t mk3 = gf16::bitmask(sh[(bit + 3) >> 5], (bit + 3) & 31);
+                U[i][a][0] = gf16::mac(U[i][a][0], m0.x0, m0.x1, m0.x2, m0.x3, mk0, mk1, mk2, mk3);
+                U[i][a][1] = gf16::mac(U[i][a][1], m1.x0, m1.x1, m1.x2, m1.x3, mk0, mk1, mk2, mk3);
+            }
+        }
+    };
+    auto fold_row = [&](int v0, int row) { fold_compute(fold_load(v0, row)); };
+    // The fold of a chunk's P rows runs during the next chunk's intervals,
+    // a share of the rows in each, so that its VALU work interleaves with
+    // the table reads; rows [f_row, f_end) of the chunk at f_v0 are pending.
+    int f_v0 = 0, f_row = 0, f_end = 0;
+    auto fold_some = [&](int nrows) {
+#pragma unroll 1
+        for (int k = 0; k < nrows && f_row < f_end; ++k, ++f_row) fold_row(f_v0, f_row);
+    };
+    // end of chunk ch: the previous chunk's fold completes, then T goes out
+    // (L rows to Mv, P rows to A.T) and the chunk's P rows become pending
+    auto chunk_end = [&](int ch) {
+        fold_some(SK_RPC);
+        WaveSync::sync();  // the fold's reads of A.T before the stores below
+        const int v0 = ch * SK_RPC, vr = v0 + slot;
+        if (active && vr < NVR) {
+            if (vr < NLR) {
+#pragma unroll
+                for (int j = 0; j < K; ++j)
+                    *(uint2 *)&Mv[(j * O + vr) * MW + 2 * g] = uint2{T[j][0], T[j][1]};
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; ++j) *(uint2 *)&A.T[(slot * K + j) * MW + 2 * g] = uint2{T[j][0], T[j][1]};
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) T[j][0] = T[j][1] = 0;
+        WaveSync::sync();
+        f_v0 = v0;
+        f_row = (v0 > NLR ? v0 : NLR) - v0;
+        f_end = (v0 + SK_RPC < NVR ? v0 + SK_RPC : NVR) - v0;
+    };
+    // the interval sequence: (chunk, first column), consumed in order; the
+    // table of interval i + 1 is produced during interval i
+    int ch = 0, c0 = cstart(0), pch = 0, pc0 = c0;
+    // entry words of the next PFD intervals to produce, fetched PFD
+    // intervals ahead (one interval lasts less than a global-memory round
+    // trip under load); (fch, fc0) is the next interval to fetch
+    constexpr int PFD = 3;
+    uint32_t xr[PFD][QPT];
+    int fch = pch, fc0 = pc0;
+    auto advance = [&]() {
+#pragma unroll
+        for (int d = 0; d + 1 < PFD; ++d)
+#pragma unroll
+            for (int u = 0; u < QPT; ++u) xr[d][u] = xr[d + 1][u];
+        fetch(fch, fc0, xr[PFD - 1]);
+        next(fch, fc0);
+        next(pch, pc0);
+    };
+#pragma unroll
+    for (int d = 0; d < PFD; ++d) {
+        fetch(fch, fc0, xr[d]);
+        next(fch, fc0);
+    }
+    produce(0, xr[0]);
+    advance();
+    if (mine) scalars(c0);
+    __syncthreads();
+    int buf = 0;
+#pragma unroll 1
+    for (;;) {
+        int nch = ch, nc0 = c0;
+        next(nch, nc0);
+        // Per interval: the first fold row's loads and column 0's table
+        // reads go out first; the next table's production and the fold
+        // row's products run while they are in flight.
+        uint2 v[K];
+        FoldIn fin;
+        int nf = 0;
+        if (mine) {
+            if (nch == ch && f_row < f_end) {
+                // pending fold rows over the intervals left in this chunk
+                const int left = (NP - c0 + CG - 1) / CG;
+                nf = (f_end - f_row + left - 1) / left;
+                fin = fold_load(f_v0, f_row);
+            }
+            issue(buf, 0, v);
+        }
+        if (pch < NCH) {
+            produce(buf ^ 1, xr[0]);
+            advance();
+        }
+        if (mine) {
+            if (nf > 0) {
+                fold_compute(fin);
+                ++f_row;
+            }
+            absorb(v);
+            if (nf > 1) fold_some(nf - 1);
+#pragma unroll
+            for (int cl = 1; cl < CG; ++cl) {
+                if (c0 + cl >= NP) break;
+                issue(buf, cl, v);
+                absorb(v);
+            }
+            if (nch < NCH) scalars(nc0);
+            if (nch != ch) chunk_end(ch);
+        }
+        ch = nch;
+        c0 = nc0;
+        __syncthreads();
+        if (ch >= NCH) break;
+        buf ^= 1;
+    }
+    if (mine) fold_some(SK_RPC);  // the last chunk's rows
+}
+
+// The pair sums of the fold items' ordered products into A.y.U (nibble
+// words): U_(a,j) + U_(j,a) for a < j, U_(a,a).
+__device__ __forceinline__ void sk_pair_sums(SkWave &A, const uint32_t (&U)[SK_IPL][K][SK_MWL]) {
+    const int lane = lane_here();
+    for (int e = lane; e < PAIRS * MW; e += 64) A.y.U[e] = 0;
+    WaveSync::sync();
+#pragma unroll
+    for (int i = 0; i < SK_IPL; ++i) {
+        const int it = lane + 64 * i;
+        if (it >= SK_NFI) continue;
+        const int j = it / SK_NWP, gp = it % SK_NWP;
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+            const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
+#pragma unroll
+            for (int h = 0; h < SK_MWL; ++h)
+                atomicXor(&A.y.U[p * MW + 2 * gp + h], gf16::planar::to_nibbles(U[i][a][h]));
+        }
+    }
+    WaveSync::sync();
+}
+
+// The workgroup's items and their keys; kernels then loop over the keys.
+template <class Acc, int W, int CG>
+__device__ __forceinline__ size_t sk_item(const Acc &acc, SkGroup<W, CG> &G, size_t base, size_t end, bool &has) {
+    const int wave = (int)uni(threadIdx.x >> 6);
+    const size_t k = base + (size_t)blockIdx.x * W + wave;
+    has = k < end;
+    const size_t item = has ? acc.order(k) : 0;
+    if ((threadIdx.x & 63) == 0) G.key[wave] = has ? (int)acc.key(item) : -1;
+    return item;
+}
+
+// Bit set w: wave w's item has key G.key of the first wave not yet done.
+template <int W, int CG>
+__device__ __forceinline__ int sk_next_key(const SkGroup<W, CG> &G, int done, int &kk) {
+    kk = -1;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (kk < 0 && !((done >> w) & 1) && G.key[w] >= 0) kk = G.key[w];
+    int sel = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (kk >= 0 && G.key[w] == kk) sel |= 1 << w;
+    return sel;
+}
+
+// Batch form over prepared keys, attempt 0, part 1 (as k_sign_prep): M, U
+// and y of positions [base, end) of the key order, W per workgroup; the
+// failure list of the part starts empty.
+template <class Acc, int CODE>
+__global__ void __launch_bounds__(64 * sk_w(CODE), 4)
+    k_sign_prep_shared(Acc acc, size_t n, int ordered, size_t base, size_t end) {
+    constexpr int W = sk_w(CODE), CG = sk_cg(CODE);
+    __shared__ SkGroup<W, CG> G;
+    const int tid = threadIdx.x, lane = tid & 63, wave = (int)uni(threadIdx.x >> 6);
+    if (blockIdx.x == 0 && tid == 0) *(uint32_t *)(acc.scratch_of(base) + SCR_LIST_COUNT) = 0;
+    bool has;
+    const size_t item = sk_item(acc, G, base, end, has);
+    SignItem it = has ? acc(item) : SignItem{};
+    SkWave &A = G.wa[wave];
+    if (has) {
+        const uint8_t *vr = it.scratch + SCR_VR;
+        for (int c = lane; c < V; c += 64) {
+            uint64_t v = 0;
+            for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(vr + j * V_BYTES, c) << (4 * j);
+            A.scol[c] = v;
+        }
+    }
+    for (int q = tid; q < sk_sstr<CG>() / 4; q += 64 * W) {
+        ((uint32_t *)G.tab[0])[q] = 0;
+        ((uint32_t *)G.tab[1])[q] = 0;
+    }
+    __syncthreads();
+    uint32_t U[SK_IPL][K][SK_MWL];
+#pragma unroll
+    for (int i = 0; i < SK_IPL; ++i)
+#pragma unroll
+        for (int a = 0; a < K; ++a) U[i][a][0] = U[i][a][1] = 0;
+    uint32_t *Mv = has ? (uint32_t *)(it.scratch + SCR_MV) : nullptr;
+    int done = 0;
+#pragma unroll 1
+    for (;;) {
+        int kk;
+        const int sel = sk_next_key(G, done, kk);
+        if (kk < 0) break;
+        quad_form_shared<O, false, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, U, Mv);
+        done |= sel;
+    }
+    if (!has) return;  // whole wavefronts; no workgroup barrier follows
+    sk_pair_sums(A, U);
+    unreduced_y<WaveSync>(A.y.U, A.y.yu);
+    if (lane == 0) {
+        uint32_t a[UW], out[MW];
+#pragma unroll
+        for (int q = 0; q < UW; ++q) a[q] = A.y.yu[q];
+        reduce_unreduced(a, out);
+        uint32_t *yv = (uint32_t *)(it.scratch + SCR_Y);
+#pragma unroll
+        for (int q = 0; q < MW; ++q) yv[q] = out[q] ^ mvec::load_mw(it.scratch + SCR_T, q);
+    }
+}
+
+// Verification over prepared keys (as k_verify_wave, t from the hashing
+// pass), positions [0, n) of the key order, W per workgroup.
+template <class Acc, int CODE>
+__global__ void __launch_bounds__(64 * sk_w(CODE), 4) k_verify_shared(Acc acc, size_t n) {
+    constexpr int W = sk_w(CODE), CG = sk_cg(CODE);
+    __shared__ SkGroup<W, CG> G;
+    const int tid = threadIdx.x, lane = tid & 63, wave = (int)uni(threadIdx.x >> 6);
+    bool has;
+    const size_t item = sk_item(acc, G, 0, n, has);
+    VerifyItem it = has ? acc(item) : VerifyItem{};
+    SkWave &A = G.wa[wave];
+    if (has) {
+        for (int c = lane; c < N; c += 64) {
+            uint64_t v = 0;
+            for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(it.sig, j * N + c) << (4 * j);
+            A.scol[c] = v;
+        }
+    }
+    for (int q = tid; q < sk_sstr<CG>() / 4; q += 64 * W) {
+        ((uint32_t *)G.tab[0])[q] = 0;
+        ((uint32_t *)G.tab[1])[q] = 0;
+    }
+    __syncthreads();
+    uint32_t U[SK_IPL][K][SK_MWL];
+#pragma unroll
+    for (int i = 0; i < SK_IPL; ++i)
+#pragma unroll
+        for (int a = 0; a < K; ++a) U[i][a][0] = U[i][a][1] = 0;
+    int done = 0;
+#pragma unroll 1
+    for (;;) {
+        int kk;
+        const int sel = sk_next_key(G, done, kk);
+        if (kk < 0) break;
+        quad_form_shared<0, true, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, U, nullptr);
+        done |= sel;
+    }
+    if (!has) return;
+    sk_pair_sums(A, U);
+    unreduced_y<WaveSync>(A.y.U, A.y.yu);
+    if (lane == 0) {
+        uint32_t a[UW], y[MW];
+#pragma unroll
+        for (int q = 0; q < UW; ++q) a[q] = A.y.yu[q];
+        reduce_unreduced(a, y);
+        bool eq = true;
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+            const int nv = mvec::word_valid(w);
+            uint32_t tw = 0;
+            for (int b = 0; b < nv; ++b) tw |= (uint32_t)it.t[4 * w + b] << (8 * b);
+            eq &= (y[w] == tw);
+        }
+        it.result[0] = eq ? 0 : 1;
+    }
+}
+
+// Shared-key variants compiled into every library: 8 items per workgroup
+// with two or one table columns (two tables of 16 KB or 8 KB per multiple
+// set), and 4 items with two columns. The prep and verification variants the
+// launch configuration accepts are the wave kernels' and these.
+inline constexpr int SK_CODES[3] = {1082, 1081, 1042};
+inline constexpr int PREP_VARIANTS[7] = {PREP_OCCS[0], PREP_OCCS[1], PREP_OCCS[2], PREP_OCCS[3],
+                                         SK_CODES[0],  SK_CODES[1],  SK_CODES[2]};
+inline constexpr int VERIFY_VARIANTS[7] = {VERIFY_OCCS[0], VERIFY_OCCS[1], VERIFY_OCCS[2], VERIFY_OCCS[3],
+                                           SK_CODES[0],    SK_CODES[1],    SK_CODES[2]};
+
+}  // namespace wv
+}  // namespace mayo

@@ -1423,8 +1423,12 @@ __global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, i
 
 // Batch form, attempt 0, part 1: M, U and y, handed over through the
 // workspace. Register-heavy (the pair accumulators of the quadratic form).
+// One item per workgroup, positions base + blockIdx.x (`end`, the part's
+// end, is implied by the grid; the shared-key variant takes the same
+// arguments).
 template <class Acc, class KL = RefKeys, int CODE = qf_code(QF_RT, QF_OCC)>
-__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0, size_t base = 0) {
+__global__ void __launch_bounds__(64, CODE % 10) k_sign_prep(Acc acc, size_t n, int ordered = 0, size_t base = 0,
+                                                             size_t end = ~(size_t)0) {
     __shared__ Shared S;
     const int lane = threadIdx.x;
     SignItem it = acc(item_of(acc, ordered, base));

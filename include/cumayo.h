@@ -351,14 +351,23 @@ int cumayo_serve_run_poisson(void *server, double rate_per_s, double duration_s,
  * batch order (0) or in an XCD-aware order (1: items sorted by key and dealt
  * to the eight XCDs in contiguous runs, so each XCD's L2 holds few keys at a
  * time; one extra one-workgroup sort per launch, batches over 64 items
- * only). Every configuration gives the same bytes. */
+ * only). sign_prep and verify also take the shared-key variants, coded
+ * 1000 + 10 W + CG: a workgroup of W wavefronts (one item each, items of one
+ * key: the items are sorted by key first, exactly for key indices below
+ * 4,096) builds the 16 GF(16) multiples of CG key columns at a time in LDS,
+ * and each item's quadratic form reads its products from those tables (the
+ * item order fields do not apply to them; with sign_chunks 0 the fused
+ * kernel runs its default quadratic form). They pay
+ * where many items share a key and the wave kernels run short of registers
+ * (MAYO_5 signing). Every configuration gives the same bytes. */
 typedef struct {
-    int32_t sign_prep;    /* quadratic form + M + y kernel (10 x rows per lane + waves per SIMD) */
+    int32_t sign_prep;    /* quadratic form + M + y kernel (10 x rows per lane + waves per SIMD,
+                             or 1000 + 10 W + CG for the shared-key tables) */
     int32_t sign_solve;   /* columns of A + elimination kernel: 100 (signatures per
                              workgroup - 1) + 10 pivot-search schedule (0 scalar
                              unit, 1 vector unit) + waves per SIMD */
     int32_t sign_order;   /* 0 batch order, 1 XCD-aware key order */
-    int32_t verify;       /* verification kernel (10 x rows per lane + waves per SIMD) */
+    int32_t verify;       /* verification kernel (as sign_prep) */
     int32_t verify_order;
     int32_t sign_chunks;  /* 1: the signing linear-algebra pass as three kernels over the
                              batch (quadratic form + M + y, elimination, retries);

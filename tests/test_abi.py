@@ -89,9 +89,14 @@ def test_launch_variants_and_config(version):
     assert cfg.as_dict() == {"sign_prep": variants["sign_prep"][0], "sign_solve": variants["sign_solve"][0],
                              "sign_order": 0, "verify": variants["verify"][0], "verify_order": 0,
                              "sign_chunks": 1}
-    # quadratic-form variants are coded 10 x rows per lane + waves per SIMD
+    # quadratic-form variants are coded 10 x rows per lane + waves per SIMD,
+    # shared-key variants 1000 + 10 x items per workgroup + table columns
     for code in variants["sign_prep"] + variants["verify"]:
-        assert code // 10 in (1, 2, 3) and 1 <= code % 10 <= 8, code
+        if code >= 1000:
+            assert (code % 1000) // 10 in (4, 8) and code % 10 in (1, 2), code
+        else:
+            assert code // 10 in (1, 2, 3) and 1 <= code % 10 <= 8, code
+    assert [c for c in variants["sign_prep"] if c >= 1000] == [c for c in variants["verify"] if c >= 1000] != []
     alt = LaunchConfig(variants["sign_prep"][1], variants["sign_solve"][2], 1, variants["verify"][1], 1, 2)
     assert lib.cumayo_set_launch_config(ctypes.byref(alt)) == 0
     got = LaunchConfig()

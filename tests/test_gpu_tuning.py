@@ -128,9 +128,17 @@ def test_autotune_keeps_a_compiled_variant_and_the_results(engine_factory, resto
     assert best["sign_prep"] in var["sign_prep"] and best["sign_solve"] in var["sign_solve"]
     assert best["verify"] in var["verify"] and best["sign_order"] in (0, 1) and best["verify_order"] in (0, 1)
     chunk_options = 1 if n < 16384 else 2  # the two-part launch applies from 16,384 items on
-    # plus the persistent fused kernel, which has no solve variant
-    assert rep["configs_timed"] == (2 * len(var["sign_prep"]) * (1 + chunk_options * len(var["sign_solve"])) +
-                                    2 * len(var["verify"]))
+    # wave-per-item variants: both item orders, plus the persistent fused
+    # kernel, which has no solve variant; shared-key variants (codes from
+    # 1000) run in their own key order and are not a fused-kernel option
+    wave_p = [p for p in var["sign_prep"] if p < 1000]
+    shared_p = [p for p in var["sign_prep"] if p >= 1000]
+    wave_v = [p for p in var["verify"] if p < 1000]
+    shared_v = [p for p in var["verify"] if p >= 1000]
+    assert shared_p and shared_v
+    assert rep["configs_timed"] == (2 * len(wave_p) * (1 + chunk_options * len(var["sign_solve"])) +
+                                    len(shared_p) * chunk_options * len(var["sign_solve"]) +
+                                    2 * len(wave_v) + len(shared_v))
     assert 0 < rep["sign_ms"] <= rep["sign_default_ms"] and 0 < rep["verify_ms"] <= rep["verify_default_ms"]
     assert torch.equal(sigs, ref)
     assert res.cpu().numpy().tolist() == [0] * n
