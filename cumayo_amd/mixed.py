@@ -4,7 +4,9 @@ Each MAYO parameter set is its own library (one build per set, as in the
 reference), so its kernels are compiled for that set's sizes. A batch that
 mixes sets (MixedBatch) is split by set on the host and each set's part runs
 through its library on a HIP stream of its own, so the sets' kernels overlap
-on the device. A mixed request stream is served by one continuous-batching
+on the device; MixedGraph captures such a pass, all four sets' pipelines on
+their forked branches, as one HIP graph that one graph launch replays. A
+mixed request stream is served by one continuous-batching
 server per set (MixedServer), all sharing the device: a request is routed to
 its set's server, the servers' batches overlap on the GPU, and latencies are
 merged. Under load, each set gets its own open-loop Poisson stream (rate x
@@ -138,6 +140,72 @@ class MixedBatch:
         for v, (pos, res) in per.items():
             results[torch.from_numpy(pos).to(dev)] = res
         return results
+
+
+class MixedGraph:
+    """The sign or verify pass of one mixed-set batch captured as one HIP
+    graph: every set's hashing and linear-algebra kernels, each set on a
+    branch forked from and joined back to the replaying stream (the
+    branches run concurrently, as MixedBatch's streams do), replayed by one
+    graph launch. The graph holds kernel nodes only (no memory-set or copy
+    nodes). The per-item arrays (key indices, offsets, lengths and, for
+    verification, the signatures) are copied into buffers the graph owns at
+    capture; messages are read from `msgs` at every replay, so the caller may
+    rewrite their bytes in place between replays (same offsets). Replays of
+    one graph must not overlap each other."""
+
+    def __init__(self, mb: MixedBatch, op: str, set_of, key_idx, msgs, msg_off, msg_len, sigs=None):
+        if op not in ("sign", "verify"):
+            raise ValueError(f"op must be 'sign' or 'verify', not {op!r}")
+        if op == "verify" and sigs is None:
+            raise ValueError("verify needs the signatures, {set: tensor} as MixedBatch.sign gives them")
+        torch = mb.torch
+        self.mb, self.op, self.msgs = mb, op, msgs
+        self.parts = mb._parts(set_of, key_idx, msg_off, msg_len)
+        self.out, self.ws, self.sigs = {}, {}, {}
+        for v, (pos, _, _, _) in self.parts.items():
+            eng = mb.engines[v]
+            n = pos.size
+            if op == "sign":
+                self.out[v] = torch.zeros((n, eng.sizes.sig_bytes), dtype=torch.uint8, device=eng.device)
+                self.ws[v] = eng.sign_workspace(n)
+            else:
+                self.sigs[v] = sigs[v].clone()
+                self.out[v] = torch.full((n,), 0xEE, dtype=torch.uint8, device=eng.device)
+                self.ws[v] = eng.verify_workspace(n)
+        dev = next(iter(mb.engines.values())).device
+        # one eager pass first: the libraries create the helper streams of
+        # the per-set streams here, outside the capture
+        self._run()
+        torch.cuda.synchronize(dev)
+        self.graph = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream(device=dev)
+        with torch.cuda.graph(self.graph, stream=cap, capture_error_mode="thread_local"):
+            self._run()
+        torch.cuda.synchronize(dev)
+
+    def _run(self):
+        def body(v, part, st):
+            eng = self.mb.engines[v]
+            _, dk, do, dl = part
+            if self.op == "sign":
+                eng.sign_prepared(self.mb.prepared[v][0], dk, self.msgs, do, dl, self.out[v], self.ws[v],
+                                  stream=st)
+            else:
+                eng.verify_prepared(self.mb.prepared[v][1], dk, self.msgs, do, dl, self.sigs[v], self.out[v],
+                                    self.ws[v], stream=st)
+
+        self.mb._launch(self.parts, body, concurrent=True)
+
+    def launch(self):
+        """One graph launch on the current stream."""
+        self.graph.replay()
+
+    def results(self) -> dict:
+        """{v: (positions, out)}: signatures (sign) or results (verify, 0
+        valid, 1 invalid) of set v's items in batch order, as of the last
+        replay."""
+        return {v: (part[0], self.out[v]) for v, part in self.parts.items()}
 
 
 class MixedServer:
