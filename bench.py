@@ -514,20 +514,22 @@ def main():
 
     comparisons = {}
     if not args.headline_only:
-        # the same pipelines captured as HIP graphs (one graph launch per pass)
-        g_sigs = torch.zeros_like(d_sigs)
-        g_res = torch.full_like(d_res, 0xEE)
-        g_ws_s, g_ws_v = eng.sign_workspace(n), eng.verify_workspace(n)
-        g_sign = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, g_sigs, g_ws_s)
-        g_verify = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, g_res, g_ws_v)
-        # untimed replays first: the graphs run over their own, freshly
-        # allocated workspaces, and their first replays run slower (the
-        # eager buffers have been through the autotuner and the warm-up)
-        for _ in range(max(args.warmup + 1, 5)):
+        # the same pipelines captured as HIP graphs (one graph launch per
+        # pass) over the eager passes' own buffers, so that the comparison is
+        # between launch mechanisms only (replays over freshly allocated
+        # signature and workspace buffers measured about 7% below the eager
+        # rate; over the same buffers the two rates agree); the eager
+        # outputs are kept to check every replay against
+        e_sigs, e_res = d_sigs.clone(), d_res.clone()
+        g_sign = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s)
+        g_verify = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_v)
+        d_sigs.fill_(0)
+        d_res.fill_(0xEE)
+        for _ in range(max(args.warmup, 2)):
             g_sign.launch()
             g_verify.launch()
         torch.cuda.synchronize(dev)
-        graph_same = bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
+        graph_same = bool(torch.equal(e_sigs, d_sigs)) and bool(torch.equal(e_res, d_res))
         t0 = time.perf_counter()
         for _ in range(args.steps):
             g_sign.launch()
@@ -538,7 +540,7 @@ def main():
             g_verify.launch()
         torch.cuda.synchronize(dev)
         graph_verify_s = (time.perf_counter() - t0) / args.steps
-        graph_same = graph_same and bool(torch.equal(g_sigs, d_sigs)) and bool(torch.equal(g_res, d_res))
+        graph_same = graph_same and bool(torch.equal(e_sigs, d_sigs)) and bool(torch.equal(e_res, d_res))
         # the eager sign pass again, right after the graph replays: the
         # graph's rate is compared with the eager rate of the same moment
         torch.cuda.synchronize(dev)
@@ -549,7 +551,7 @@ def main():
         eager_after_s = (time.perf_counter() - t0) / args.steps
         g_sign.close()
         g_verify.close()
-        del g_sigs, g_res, g_ws_s, g_ws_v
+        del e_sigs, e_res
 
         # the plain form: one workgroup per item running the whole signing
         # (or verification) stage in one kernel, the reference's own structure
