@@ -681,9 +681,9 @@ def main():
                 "vgpr_mask_cycles_per_wave_op": meas.get("vgpr_mask_cycles_per_wave_op")}
 
     if not args.headline_only and os.path.exists(_build.contraction_path()):
-        # the shared-key P1 contraction in its bitsliced VALU form and its
-        # int8-MFMA GF(2)-plane form, same data, both checked against a CPU
-        # restatement (cumayo_amd/csrc/contraction_bench.hip)
+        # the shared-key P1 contraction in its bitsliced VALU form, its
+        # int8-MFMA GF(2)-plane form and its LDS-table form, same data, all
+        # checked against a CPU restatement (cumayo_amd/csrc/contraction_bench.hip)
         cb = subprocess.run([_build.contraction_path()], capture_output=True, text=True, timeout=120)
         if cb.returncode == 0 and cb.stdout.strip():
             out["mfma_contraction"] = json.loads(cb.stdout.strip().splitlines()[-1])

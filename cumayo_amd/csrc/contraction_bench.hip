@@ -20,11 +20,19 @@
 //    workgroup with two tiles per wave runs one workgroup per CU and took
 //    twice as long.)
 //
-// Both kernels reduce what they compute into per-wave checksums, and both
+//  * LDS tables: a workgroup of TW = 8 signatures of one key builds, in LDS,
+//    the 16 multiples s P1[r][c] (s = 0..15) of the entries of two columns
+//    of a 12-row chunk at a time (double-buffered, one barrier per table),
+//    and each signature's wavefront takes every product S_j[c] P1[r][c] as
+//    one 8-byte LDS read at an address set by its (uniform) scalar, plus
+//    two XORs; lanes = (word pair, row), one row per lane and chunk. This is
+//    the column loop of the shared-key signing kernels, without their fold.
+//
+// All kernels reduce what they compute into per-wave checksums, and all
 // write T in full for the first 16 signatures, which the program compares
 // with a CPU restatement (the reference's loops). MAYO_1 sizes (v = m = 78,
 // k = 10): 65,536 signatures under 64 keys, 1,024 signatures per key. Prints
-// one JSON line with both kernels' times and the correctness check.
+// one JSON line with the kernels' times and the correctness check.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -253,6 +261,132 @@ __global__ void __launch_bounds__(64 * WAVES) k_mfma(const v4i *__restrict__ abl
     for (int u = 0; u < TPW; ++u) cs_out[((size_t)blockIdx.x * WAVES + wave) * 128 + u * 64 + lane] = cs[u];
 }
 
+// ---- LDS-table kernel ----
+constexpr int TW = 8;                       // signatures per workgroup (one key)
+constexpr int TWP = MW / 2;                 // word pairs per m-vector (5)
+constexpr int TRPC = 64 / TWP;              // rows per chunk, one per lane slot (12)
+constexpr int TNCH = (VV + TRPC - 1) / TRPC;
+constexpr int TCG = 2;                      // columns per table
+constexpr int TCSTR = TRPC * MW * 4;        // bytes of one column's entries (480)
+constexpr int TSSTR = 1024;                 // bytes per multiple plane (>= TCG * TCSTR + idle lanes)
+constexpr int TBUF = 16 * TSSTR;            // one table: 16 planes
+constexpr int TNQ = TRPC * TCG * MW;        // entry words per table (240)
+static_assert(TCG * TCSTR + 8 * (64 - TRPC * TWP) <= TSSTR, "idle lanes read inside the plane");
+static_assert(TNQ <= 64 * TW, "one word per producing thread");
+
+__device__ __forceinline__ void tab_next(int &ch, int &c0) {
+    c0 += TCG;
+    if (c0 >= VV) {
+        ++ch;
+        c0 = ch * TRPC;
+    }
+}
+
+__global__ void __launch_bounds__(64 * TW) k_ldst(const uint32_t *__restrict__ p1,
+                                                  const uint64_t *__restrict__ scol_all,
+                                                  uint32_t *__restrict__ cs_out, uint8_t *__restrict__ t_out) {
+    __shared__ __attribute__((aligned(16))) uint8_t tab[2 * TBUF];
+    __shared__ uint64_t scol_s[TW][VV + TCG];
+    const int tid = threadIdx.x, lane = tid & 63, wave = (int)__builtin_amdgcn_readfirstlane(tid >> 6);
+    const int s0 = blockIdx.x * TW, key = s0 / SIGS_PER_KEY;
+    const int sig = s0 + wave;
+    const uint32_t *P = p1 + (size_t)key * P1E * MW;
+    uint64_t *scol = scol_s[wave];
+    for (int c = lane; c < VV + TCG; c += 64) scol[c] = c < VV ? scol_all[(size_t)sig * VV + c] : 0;
+    for (int q = tid; q < TSSTR / 4; q += 64 * TW) {  // the zero multiple of both tables
+        ((uint32_t *)tab)[q] = 0;
+        ((uint32_t *)(tab + TBUF))[q] = 0;
+    }
+    // consumer lane: word pair g of row slot `slot`; producer thread: word q
+    const int g = lane % TWP, slot = lane / TWP;
+    const bool active = slot < TRPC;
+    const int qcl = tid / (TRPC * MW), qrow = (tid / MW) % TRPC, qw = tid % MW;
+    auto fetch = [&](int ch, int c0) -> uint32_t {
+        const int r = ch * TRPC + qrow, c = c0 + qcl;
+        return tid < TNQ && ch < TNCH && r < VV && c < VV && c >= r ? P[tri(r, c) * MW + qw] : 0u;
+    };
+    auto produce = [&](int buf, uint32_t x1) {
+        if (tid >= TNQ) return;
+        using namespace gf16::planar;
+        const uint32_t x2 = mul_a(x1), x4 = mul_a(x2), x8 = mul_a(x4);
+        const uint32_t x3 = x1 ^ x2, x5 = x1 ^ x4, x6 = x2 ^ x4, x7 = x3 ^ x4;
+        uint32_t *d = (uint32_t *)(tab + buf * TBUF) + tid;
+        constexpr int S4 = TSSTR / 4;
+        d[1 * S4] = x1; d[2 * S4] = x2; d[3 * S4] = x3; d[4 * S4] = x4;
+        d[5 * S4] = x5; d[6 * S4] = x6; d[7 * S4] = x7; d[8 * S4] = x8;
+        d[9 * S4] = x1 ^ x8; d[10 * S4] = x2 ^ x8; d[11 * S4] = x3 ^ x8; d[12 * S4] = x4 ^ x8;
+        d[13 * S4] = x5 ^ x8; d[14 * S4] = x6 ^ x8; d[15 * S4] = x7 ^ x8;
+    };
+    uint32_t acc[KK][2];
+#pragma unroll
+    for (int j = 0; j < KK; ++j) acc[j][0] = acc[j][1] = 0;
+    uint32_t cs = 0;
+    int ch = 0, c0 = 0, pch = 0, pc0 = 0;
+    produce(0, fetch(0, 0));
+    tab_next(pch, pc0);
+    uint32_t x = fetch(pch, pc0);
+    __syncthreads();
+    int buf = 0;
+#pragma unroll 1
+    for (;;) {
+        if (pch < TNCH) {
+            produce(buf ^ 1, x);
+            tab_next(pch, pc0);
+            x = fetch(pch, pc0);
+        }
+        const uint8_t *bb = tab + buf * TBUF + slot * MW * 4 + g * 8;
+        uint64_t scv[TCG];
+#pragma unroll
+        for (int cl = 0; cl < TCG; ++cl) scv[cl] = scol[c0 + cl];
+#pragma unroll
+        for (int cl = 0; cl < TCG; ++cl) {
+            if (c0 + cl >= VV) break;
+            const uint64_t sc = scv[cl];
+            const uint32_t sh[2] = {(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)sc),
+                                    (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(sc >> 32))};
+#pragma unroll
+            for (int j = 0; j < KK; ++j) {
+                const int bit = 4 * j;
+                const uint32_t s = bit < 32 ? ((sh[0] >> bit) | (bit > 28 ? sh[1] << (32 - bit) : 0u)) & 15u
+                                            : (sh[1] >> (bit - 32)) & 15u;
+                const uint2 v = *(const uint2 *)(bb + s * TSSTR + cl * TCSTR);
+                acc[j][0] ^= v.x;
+                acc[j][1] ^= v.y;
+            }
+        }
+        int nch = ch, nc0 = c0;
+        tab_next(nch, nc0);
+        if (nch != ch) {
+            // chunk done: row ch * TRPC + slot of every T_j, words 2g, 2g + 1
+            const int r = ch * TRPC + slot;
+#pragma unroll
+            for (int j = 0; j < KK; ++j) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t T = acc[j][h];
+                    cs ^= __builtin_amdgcn_alignbit(T, T, (uint32_t)(r + j + h) & 31u);
+                    const int w = 2 * g + h;
+                    if (sig < CHECK && active && r < VV) {
+                        for (int q = 0; q < 8 && 8 * w + q < MM; ++q) {
+                            uint32_t nib = 0;
+#pragma unroll
+                            for (int d = 0; d < 4; ++d) nib |= ((T >> (8 * d + q)) & 1u) << d;
+                            t_out[(((size_t)sig * KK + j) * VV + r) * MM + 8 * w + q] = (uint8_t)nib;
+                        }
+                    }
+                    acc[j][h] = 0;
+                }
+            }
+        }
+        ch = nch;
+        c0 = nc0;
+        __syncthreads();
+        if (ch >= TNCH) break;
+        buf ^= 1;
+    }
+    cs_out[(size_t)sig * 64 + lane] = cs;
+}
+
 // ---- host: data, CPU restatement, layouts ----
 uint32_t rng_state = 0x9E3779B9u;
 uint32_t rnd() {
@@ -376,22 +510,34 @@ int main() {
     const float mfma_ms = time_ms([&] {
         k_mfma<<<mblocks, 64 * WAVES>>>((const v4i *)d_ab, d_boff, bpk, d_vn, d_csm, (uint32_t *)d_tm);
     });
+    uint8_t *d_tl;
+    uint32_t *d_csl;
+    if (hipMalloc(&d_tl, tbytes) || hipMalloc(&d_csl, (size_t)NSIG * 64 * 4)) {
+        printf("{\"error\": \"allocation\"}\n");
+        return 1;
+    }
+    (void)hipMemset(d_tl, 0, tbytes);
+    const float lds_ms = time_ms([&] { k_ldst<<<NSIG / TW, 64 * TW>>>(d_p1w, d_scol, d_csl, d_tl); });
     const int err = (int)hipDeviceSynchronize() | (int)hipGetLastError();
-    std::vector<uint8_t> tv(tbytes);
+    std::vector<uint8_t> tv(tbytes), tl(tbytes);
+    (void)hipMemcpy(tl.data(), d_tl, tbytes, hipMemcpyDeviceToHost);
     std::vector<uint32_t> tm(tbytes);
     (void)hipMemcpy(tv.data(), d_tv, tbytes, hipMemcpyDeviceToHost);
     (void)hipMemcpy(tm.data(), d_tm, tbytes * 4, hipMemcpyDeviceToHost);
-    size_t bad_v = 0, bad_m = 0;
+    size_t bad_v = 0, bad_m = 0, bad_l = 0;
     for (size_t x = 0; x < tbytes; ++x) {
         bad_v += tv[x] != tref[x];
         bad_m += (tm[x] & 0xFu) != tref[x];  // each of the four timed runs ORs the same bits in
+        bad_l += tl[x] != tref[x];
     }
     const double macs = (double)NSIG * P1E * KK * MM;  // GF(16) MACs of the contraction
     printf("{\"contraction\": \"T_j[r] = sum_c P1[r][c] V_j[c], MAYO_1, %d signatures, %d keys\", "
-           "\"valu_ms\": %.4f, \"mfma_ms\": %.4f, \"valu_tmacs\": %.2f, \"mfma_tmacs\": %.2f, "
+           "\"valu_ms\": %.4f, \"mfma_ms\": %.4f, \"lds_table_ms\": %.4f, \"valu_tmacs\": %.2f, "
+           "\"mfma_tmacs\": %.2f, \"lds_table_tmacs\": %.2f, "
            "\"mfma_instructions_per_signature\": %.1f, \"checked_signatures\": %d, "
-           "\"valu_mismatches\": %zu, \"mfma_mismatches\": %zu, \"hip_error\": %d}\n",
-           NSIG, NKEY, valu_ms, mfma_ms, macs / (valu_ms * 1e-3) / 1e12, macs / (mfma_ms * 1e-3) / 1e12,
-           (double)bpk * NTILES / G, CHECK, bad_v, bad_m, err);
-    return err != 0 || bad_v || bad_m;
+           "\"valu_mismatches\": %zu, \"mfma_mismatches\": %zu, \"lds_table_mismatches\": %zu, "
+           "\"hip_error\": %d}\n",
+           NSIG, NKEY, valu_ms, mfma_ms, lds_ms, macs / (valu_ms * 1e-3) / 1e12, macs / (mfma_ms * 1e-3) / 1e12,
+           macs / (lds_ms * 1e-3) / 1e12, (double)bpk * NTILES / G, CHECK, bad_v, bad_m, bad_l, err);
+    return err != 0 || bad_v || bad_m || bad_l;
 }
