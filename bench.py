@@ -172,6 +172,8 @@ def mixed_batch_run(eng1, n, steps, weights=None, keys=64):
     rates = {}
     outs = {}
     for conc in (True, False):
+        print(f"[bench] mixed-set batch: {'sets side by side' if conc else 'sets in sequence'}", file=sys.stderr,
+              flush=True)
         sig = mb.sign(set_of, kidx, d_msgs, offs, lens, concurrent=conc)
         res = mb.verify(set_of, kidx, d_msgs, offs, lens, {v: s for v, (p, s) in sig.items()}, concurrent=conc)
         torch.cuda.synchronize(dev)
@@ -189,34 +191,11 @@ def mixed_batch_run(eng1, n, steps, weights=None, keys=64):
         rates[conc] = (n / ts, n / tv)
         outs[conc] = ({v: s.cpu().numpy() for v, (p, s) in sig.items()}, res.cpu().numpy())
     same = all(np.array_equal(outs[True][0][v], outs[False][0][v]) for v in outs[True][0])
-    # the same two passes as one HIP graph each (every set's pipeline on its
-    # own branch), one graph launch per pass
-    from cumayo_amd.mixed import MixedGraph
-    gs = MixedGraph(mb, "sign", set_of, kidx, d_msgs, offs, lens)
-    gv = MixedGraph(mb, "verify", set_of, kidx, d_msgs, offs, lens, {v: s for v, (p, s) in sig.items()})
-    for _ in range(2):
-        gs.launch()
-        gv.launch()
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        gs.launch()
-    torch.cuda.synchronize(dev)
-    tgs = (time.perf_counter() - t0) / steps
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        gv.launch()
-    torch.cuda.synchronize(dev)
-    tgv = (time.perf_counter() - t0) / steps
-    graph_same = all(np.array_equal(o.cpu().numpy(), outs[True][0][v]) for v, (p, o) in gs.results().items())
-    graph_valid = all(bool((o.cpu().numpy() == 0).all()) for v, (p, o) in gv.results().items())
     return {"items": n, "mix": {f"MAYO_{v}": int((set_of == v).sum()) for v in weights},
             "messages": "uniform [32, 1024] B, key j mod 64 of the item's set",
             "sign_per_s": round(rates[True][0], 1), "verify_per_s": round(rates[True][1], 1),
             "sets_in_sequence": {"sign_per_s": round(rates[False][0], 1),
                                  "verify_per_s": round(rates[False][1], 1)},
-            "one_graph_launch": {"sign_per_s": round(n / tgs, 1), "verify_per_s": round(n / tgv, 1),
-                                 "identical_outputs": bool(graph_same), "all_valid": bool(graph_valid)},
             "identical_outputs": bool(same), "all_valid": bool((outs[True][1] == 0).all())}
 
 
@@ -516,10 +495,8 @@ def main():
     if not args.headline_only:
         # the same pipelines captured as HIP graphs (one graph launch per
         # pass) over the eager passes' own buffers, so that the comparison is
-        # between launch mechanisms only (replays over freshly allocated
-        # signature and workspace buffers measured about 7% below the eager
-        # rate; over the same buffers the two rates agree); the eager
-        # outputs are kept to check every replay against
+        # between launch mechanisms only; the eager outputs are kept to check
+        # every replay against
         e_sigs, e_res = d_sigs.clone(), d_res.clone()
         g_sign = eng.sign_graph(p_esk, d_kidx, d_msgs, d_off, d_len, d_sigs, ws_s)
         g_verify = eng.verify_graph(p_epk, d_kidx, d_msgs, d_off, d_len, d_sigs, d_res, ws_v)

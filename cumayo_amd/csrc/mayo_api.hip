@@ -1113,6 +1113,32 @@ int cumayo_graph_launch(cumayo_graph *g, hipStream_t stream) {
     return (int)e;
 }
 
+cumayo_graph *cumayo_graph_compose(cumayo_graph *const *parts, int n) {
+    if (!parts || n <= 0) {
+        note(hipErrorInvalidValue);
+        return nullptr;
+    }
+    for (int i = 0; i < n; ++i)
+        if (!parts[i] || !parts[i]->graph) {
+            note(hipErrorInvalidValue);
+            return nullptr;
+        }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddChildGraphNode(&node, g, nullptr, 0, parts[i]->graph);
+    }
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        note(e);
+        return nullptr;
+    }
+    return new cumayo_graph{g, x};
+}
+
 void cumayo_graph_destroy(cumayo_graph *g) {
     if (!g) return;
     (void)hipGraphExecDestroy(g->exec);

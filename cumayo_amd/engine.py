@@ -141,6 +141,7 @@ EXPORTS = {
     "cumayo_verify_graph_create": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, ctypes.c_int],
     "cumayo_graph_launch": [_VP, _VP],
     "cumayo_graph_destroy": [_VP],
+    "cumayo_graph_compose": [_VP, ctypes.c_int],
 }
 _RESTYPES = {
     "cumayo_last_error": ctypes.c_int,
@@ -168,6 +169,7 @@ _RESTYPES = {
     "cumayo_sign_graph_create": _VP,
     "cumayo_verify_graph_create": _VP,
     "cumayo_graph_launch": ctypes.c_int,
+    "cumayo_graph_compose": _VP,
 }
 
 
@@ -568,6 +570,16 @@ class Engine:
                                                 stages)
         if not h:
             raise RuntimeError(f"cumayo_verify_graph_create failed: HIP error {self.lib.cumayo_last_error()}")
+        return Graph(self, h)
+
+
+    def compose_graphs(self, graphs) -> "Graph":
+        """One graph running the given pipelines (Graph objects of any
+        parameter set's engine) as independent children: one launch."""
+        arr = (ctypes.c_void_p * len(graphs))(*[g.handle for g in graphs])
+        h = self.lib.cumayo_graph_compose(ctypes.cast(arr, ctypes.c_void_p), len(graphs))
+        if not h:
+            raise RuntimeError(f"cumayo_graph_compose failed: HIP error {self.lib.cumayo_last_error()}")
         return Graph(self, h)
 
 

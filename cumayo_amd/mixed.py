@@ -4,9 +4,9 @@ Each MAYO parameter set is its own library (one build per set, as in the
 reference), so its kernels are compiled for that set's sizes. A batch that
 mixes sets (MixedBatch) is split by set on the host and each set's part runs
 through its library on a HIP stream of its own, so the sets' kernels overlap
-on the device; MixedGraph captures such a pass, all four sets' pipelines on
-their forked branches, as one HIP graph that one graph launch replays. A
-mixed request stream is served by one continuous-batching
+on the device; MixedGraph makes such a pass one HIP graph (each set's
+pipeline, captured by its own library, a child of one graph), so that one
+graph launch runs it. A mixed request stream is served by one continuous-batching
 server per set (MixedServer), all sharing the device: a request is routed to
 its set's server, the servers' batches overlap on the GPU, and latencies are
 merged. Under load, each set gets its own open-loop Poisson stream (rate x
@@ -143,16 +143,17 @@ class MixedBatch:
 
 
 class MixedGraph:
-    """The sign or verify pass of one mixed-set batch captured as one HIP
-    graph: every set's hashing and linear-algebra kernels, each set on a
-    branch forked from and joined back to the replaying stream (the
-    branches run concurrently, as MixedBatch's streams do), replayed by one
-    graph launch. The graph holds kernel nodes only (no memory-set or copy
-    nodes). The per-item arrays (key indices, offsets, lengths and, for
-    verification, the signatures) are copied into buffers the graph owns at
-    capture; messages are read from `msgs` at every replay, so the caller may
-    rewrite their bytes in place between replays (same offsets). Replays of
-    one graph must not overlap each other."""
+    """The sign or verify pass of one mixed-set batch as one HIP graph: each
+    set's pipeline (hashing, linear-algebra and retry kernels) is captured by
+    its own library (cumayo_sign_graph_create / cumayo_verify_graph_create)
+    over buffers this object owns, and the pipelines become the independent
+    children of one graph (cumayo_graph_compose), so one graph launch runs
+    every set and the sets' kernels may overlap on the device. The graphs
+    hold kernel nodes only (no memory-set or copy nodes). The per-item arrays
+    (key indices, offsets, lengths and, for verification, the signatures) are
+    copied into the owned buffers at construction; messages are read from
+    `msgs` at every launch, so the caller may rewrite their bytes in place
+    between launches (same offsets). Launches must not overlap each other."""
 
     def __init__(self, mb: MixedBatch, op: str, set_of, key_idx, msgs, msg_off, msg_len, sigs=None):
         if op not in ("sign", "verify"):
@@ -162,50 +163,37 @@ class MixedGraph:
         torch = mb.torch
         self.mb, self.op, self.msgs = mb, op, msgs
         self.parts = mb._parts(set_of, key_idx, msg_off, msg_len)
-        self.out, self.ws, self.sigs = {}, {}, {}
-        for v, (pos, _, _, _) in self.parts.items():
+        self.out, self.ws, self.sigs, self.graphs = {}, {}, {}, {}
+        for v, (pos, dk, do, dl) in self.parts.items():
             eng = mb.engines[v]
             n = pos.size
             if op == "sign":
                 self.out[v] = torch.zeros((n, eng.sizes.sig_bytes), dtype=torch.uint8, device=eng.device)
                 self.ws[v] = eng.sign_workspace(n)
+                self.graphs[v] = eng.sign_graph(mb.prepared[v][0], dk, msgs, do, dl, self.out[v], self.ws[v])
             else:
                 self.sigs[v] = sigs[v].clone()
                 self.out[v] = torch.full((n,), 0xEE, dtype=torch.uint8, device=eng.device)
                 self.ws[v] = eng.verify_workspace(n)
-        dev = next(iter(mb.engines.values())).device
-        # one eager pass first: the libraries create the helper streams of
-        # the per-set streams here, outside the capture
-        self._run()
-        torch.cuda.synchronize(dev)
-        self.graph = torch.cuda.CUDAGraph()
-        cap = torch.cuda.Stream(device=dev)
-        with torch.cuda.graph(self.graph, stream=cap, capture_error_mode="thread_local"):
-            self._run()
-        torch.cuda.synchronize(dev)
+                self.graphs[v] = eng.verify_graph(mb.prepared[v][1], dk, msgs, do, dl, self.sigs[v], self.out[v],
+                                                  self.ws[v])
+        first = mb.engines[next(iter(self.graphs))]
+        self.graph = first.compose_graphs(list(self.graphs.values()))
 
-    def _run(self):
-        def body(v, part, st):
-            eng = self.mb.engines[v]
-            _, dk, do, dl = part
-            if self.op == "sign":
-                eng.sign_prepared(self.mb.prepared[v][0], dk, self.msgs, do, dl, self.out[v], self.ws[v],
-                                  stream=st)
-            else:
-                eng.verify_prepared(self.mb.prepared[v][1], dk, self.msgs, do, dl, self.sigs[v], self.out[v],
-                                    self.ws[v], stream=st)
-
-        self.mb._launch(self.parts, body, concurrent=True)
-
-    def launch(self):
-        """One graph launch on the current stream."""
-        self.graph.replay()
+    def launch(self, stream=None):
+        """One graph launch on `stream` (the current stream by default)."""
+        self.graph.launch(stream)
 
     def results(self) -> dict:
         """{v: (positions, out)}: signatures (sign) or results (verify, 0
         valid, 1 invalid) of set v's items in batch order, as of the last
-        replay."""
+        launch."""
         return {v: (part[0], self.out[v]) for v, part in self.parts.items()}
+
+    def close(self):
+        self.graph.close()
+        for g in self.graphs.values():
+            g.close()
 
 
 class MixedServer:

@@ -431,6 +431,15 @@ cumayo_graph *cumayo_verify_graph_create(const uint8_t *prepared, const int32_t 
                                          uint8_t *results, size_t n, uint8_t *workspace, int stages);
 int cumayo_graph_launch(cumayo_graph *g, hipStream_t stream);
 void cumayo_graph_destroy(cumayo_graph *g);
+/* One graph whose n child graphs are the pipelines `parts` (created by this
+ * or another parameter set's library: the graph object is the same in each),
+ * with no dependencies between them, so one cumayo_graph_launch runs them
+ * all and they may overlap on the device: a batch that mixes parameter sets
+ * as one graph launch. The children are copies; the parts stay valid and
+ * are destroyed by their owner. NULL on invalid arguments or a HIP error
+ * (cumayo_last_error). The parts' buffers must not be shared between
+ * them. */
+cumayo_graph *cumayo_graph_compose(cumayo_graph *const *parts, int n);
 
 #ifdef __cplusplus
 }

@@ -70,10 +70,12 @@ def test_mixed_set_batch_equals_single_set_batches(engine_factory):
 
 
 def test_mixed_set_graph_replays_equal_eager(engine_factory):
-    """One HIP graph over a four-set batch (MixedGraph): every replay, with
-    the program's own eager work and synchronisations between replays, gives
-    the eager MixedBatch bytes; the graph reads the messages at replay time;
-    a verification graph accepts the signatures and rejects mutated ones."""
+    """One HIP graph over a four-set batch (MixedGraph: each set's pipeline
+    captured by its library, composed as children of one graph): every
+    replay, with the program's own eager work and synchronisations between
+    replays, gives the eager MixedBatch bytes; the graph reads the messages
+    at replay time; a verification graph accepts the signatures and rejects
+    mutated ones."""
     import torch
     from cumayo_amd.mixed import MixedBatch, MixedGraph
     sets = (1, 2, 3, 5)
@@ -150,3 +152,58 @@ def test_mixed_set_graph_replays_equal_eager(engine_factory):
         res = mb.verify(set_of, kidx, d_msgs, offs, lens, sigs)
         torch.cuda.synchronize()
         assert res.cpu().numpy().tolist() == [0] * n
+    for graph in (g, gv, gb):
+        graph.close()
+
+
+def test_mixed_set_graph_with_two_part_launch(engine_factory):
+    """The composed graph with MAYO_1's part above the two-part threshold
+    (16,384 items) and the two-part launch in force for it: replays equal
+    the eager bytes, and the verification graph accepts every signature."""
+    import torch
+    from cumayo_amd.mixed import MixedBatch, MixedGraph
+    sets = (1, 2, 3, 5)
+    engines, prepared = {}, {}
+    for v in sets:
+        eng = engine_factory(v)
+        esk, epk = eng.keypair_tables(4, rand_index=90 + v)
+        engines[v], prepared[v] = eng, eng.prepare_tables(esk, epk)
+    saved = engines[1].launch_config()
+    engines[1].set_launch_config(sign_chunks=2)
+    try:
+        n = 48000
+        rng = np.random.default_rng(4242)
+        set_of = rng.choice(sets, size=n, p=[0.4, 0.2, 0.2, 0.2]).astype(np.int32)
+        assert (set_of == 1).sum() >= 16384
+        lens = rng.integers(0, 300, n).astype(np.int32)
+        offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+        flat = rng.integers(0, 256, int(lens.sum()) + 1, dtype=np.uint8)
+        kidx = rng.integers(0, 4, n).astype(np.int32)
+        dev = engines[1].device
+        d_msgs = torch.from_numpy(flat).to(dev)
+        mb = MixedBatch(engines, prepared)
+        print(f"[mixed-graph-2part] n={n} sets {np.bincount(set_of).tolist()}", flush=True)
+        ref = mb.sign(set_of, kidx, d_msgs, offs, lens)
+        torch.cuda.synchronize()
+        print("[mixed-graph-2part] eager done, composing", flush=True)
+        g = MixedGraph(mb, "sign", set_of, kidx, d_msgs, offs, lens)
+        gv = MixedGraph(mb, "verify", set_of, kidx, d_msgs, offs, lens, {v: s for v, (p, s) in ref.items()})
+        for rep in range(2):
+            for _, out in g.results().values():
+                out.fill_(0)
+            torch.cuda.synchronize()
+            g.launch()
+            gv.launch()
+            torch.cuda.synchronize()
+            print(f"[mixed-graph-2part] replay {rep} done", flush=True)
+            for v, (pos, out) in g.results().items():
+                assert torch.equal(out, ref[v][1]), (rep, v)
+            for v, (pos, out) in gv.results().items():
+                assert out.cpu().numpy().tolist() == [0] * pos.size, (rep, v)
+            again = mb.sign(set_of, kidx, d_msgs, offs, lens)
+            torch.cuda.synchronize()
+            assert all(torch.equal(again[v][1], ref[v][1]) for v in sets)
+        g.close()
+        gv.close()
+    finally:
+        engines[1].set_launch_config(**saved)
