@@ -217,6 +217,20 @@ struct Shake {
             if (w % RW == RW - 1 && 8 * w + 7 < POS + LEN) f1600(st);
         }
     }
+    // Absorb NW 64-bit words (little-endian byte order) as the input bytes
+    // [POS, POS + 8 NW), POS a multiple of 8, permuting after each full
+    // block: inputs held in registers (digests, salts, seeds).
+    template <int POS, int NW>
+    __device__ __forceinline__ void absorb_words_at(const uint64_t *x) {
+        static_assert(POS % 8 == 0, "word-aligned position");
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            constexpr int w0 = POS / 8;
+            const int w = w0 + k;
+            st[w % RW] ^= x[k];
+            if (w % RW == RW - 1) f1600(st);
+        }
+    }
     // One byte b at stream position POS.
     template <int POS>
     __device__ __forceinline__ void absorb_byte_at(uint32_t b) {
@@ -233,6 +247,14 @@ struct Shake {
         f1600(st);
     }
 
+    // The first NW words of the output (a single call after finalize, NW
+    // within the rate) into registers.
+    template <int NW>
+    __device__ __forceinline__ void squeeze_words(uint64_t *x) const {
+        static_assert(NW <= RW, "one block");
+#pragma unroll
+        for (int k = 0; k < NW; ++k) x[k] = st[k];
+    }
     // Squeeze len bytes (a single call after finalize) into out.
     __device__ void squeeze(uint8_t *out, int len) {
         for (int base = 0;; base += RATE) {

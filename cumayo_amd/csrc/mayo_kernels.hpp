@@ -320,27 +320,40 @@ __global__ void __launch_bounds__(64) k_sign_hash(Acc acc, size_t n) {
     if constexpr (ORDERED) i = acc.order(i);
     SignItem it = acc(i);
     uint8_t *s = it.scratch;
+    // the digest, salt and seed stay in registers between the four hashes
+    // (each also goes to the workspace for the later kernels, but is never
+    // read back from it here: a store-to-load round trip through memory per
+    // hash, with one wave per SIMD, stalled the whole pass)
+    static_assert(DIGEST % 8 == 0 && SK % 8 == 0, "whole words");
+    constexpr int DW = DIGEST / 8, SW = SK / 8;
+    uint64_t dig[DW], salt[SW], seed[SW];
+#pragma unroll
+    for (int k = 0; k < SW; ++k) seed[k] = keccak::Shake<136>::load64(it.esk + 8 * k);
     keccak::Shake<136> h;
     h.init();
     h.finalize_at(h.absorb_msg(it.msg, it.mlen));
+    h.squeeze_words<DW>(dig);
     h.squeeze(s + SCR_DIGEST, DIGEST);
     // salt = SHAKE256(digest || R || seed_sk), R = 0 (absorbing zeros is a
     // no-op; no block ends inside R, so no permutation is skipped with it)
     static_assert((DIGEST + SK) <= 136, "R lies in the first block");
     h.init();
-    h.absorb_at<0, DIGEST>(s + SCR_DIGEST);
-    h.absorb_at<DIGEST + SK, SK>(it.esk);
+    h.absorb_words_at<0, DW>(dig);
+    h.absorb_words_at<DIGEST + SK, SW>(seed);
     h.finalize_static<DIGEST + 2 * SK>();
+    h.squeeze_words<SW>(salt);
     h.squeeze(s + SCR_SALT, SK);
     // t = SHAKE256(digest || salt)
     h.init();
-    h.absorb_at<0, DIGEST + SK>(s + SCR_DIGEST);
+    h.absorb_words_at<0, DW>(dig);
+    h.absorb_words_at<DIGEST, SW>(salt);
     h.finalize_static<DIGEST + SK>();
     h.squeeze(s + SCR_T, MB);
     // V || r = SHAKE256(digest || salt || seed_sk || ctr), ctr = 0
     h.init();
-    h.absorb_at<0, DIGEST + SK>(s + SCR_DIGEST);
-    h.absorb_at<DIGEST + SK, SK>(it.esk);
+    h.absorb_words_at<0, DW>(dig);
+    h.absorb_words_at<DIGEST, SW>(salt);
+    h.absorb_words_at<DIGEST + SK, SW>(seed);
     h.finalize_static<DIGEST + 2 * SK + 1>();
     h.squeeze(s + SCR_VR, VR_BYTES);
 }
