@@ -418,7 +418,7 @@ void cumayo_test_pitched_fail_first(int attempts) {
     g_pitched_fail_first.store(attempts > 0 ? (attempts < MAX_ATTEMPTS ? attempts : MAX_ATTEMPTS) : 0);
 }
 
-size_t cumayo_sign_workspace_size(size_t n) { return n * (size_t)SIGN_SCRATCH_STRIDE; }
+size_t cumayo_sign_workspace_size(size_t n) { return n * ((size_t)SIGN_SCRATCH_STRIDE + SIGN_T_BYTES); }
 
 }  // extern "C"
 
@@ -487,20 +487,27 @@ hipStream_t aux_stream(hipStream_t caller) {
 }
 
 // Stage 2 of signing: attempt 0 in two kernels tuned for their register
-// needs (prep_k, solve_k), then the retry pass over the items whose attempt
-// met a singular system (about 1 in 4,000). Over the whole batch, or, with
+// needs (prep_k, solve_k; with the shared-key tables prep_k writes T rows
+// and fold_k folds them into y between the two), then the retry pass over the
+// items whose attempt met a singular system (about 1 in 4,000). Over the
+// whole batch, or, with
 // parts = 2, over two halves, the second on the helper stream after a fork
 // from `stream` and joined back at the end: the kernels of one half fill the
 // other half's tails, and each half's retry pass (over the half's own
 // failure list) runs beside the other half's kernels.
+using FoldK = void (*)(TableSign, size_t, size_t, size_t);
+
 template <class KL, class PK, class SK>
-hipError_t launch_sign_stage2(PK prep_k, int prep_w, SK solve_k, int solve_w, const TableSign &acc, size_t n,
-                              int ordered, int fail_first, int parts, hipStream_t stream) {
+hipError_t launch_sign_stage2(PK prep_k, int prep_w, FoldK fold_k, SK solve_k, int solve_w, const TableSign &acc,
+                              size_t n, int ordered, int fail_first, int parts, hipStream_t stream) {
     const unsigned retry_threads = 64 * wv::RNW;
-    // prep, solve and retry over the positions [base, base + m) on s
+    // prep, (fold,) solve and retry over the positions [base, base + m) on s
     auto part = [&](size_t base, size_t m, hipStream_t s) {
         hipLaunchKernelGGL(prep_k, dim3((unsigned)((m + prep_w - 1) / prep_w)), dim3(64 * prep_w), 0, s, acc, n,
                            ordered, base, base + m);
+        if (fold_k)
+            hipLaunchKernelGGL(fold_k, dim3((unsigned)((m + wv::FOLD_W - 1) / wv::FOLD_W)), dim3(64 * wv::FOLD_W),
+                               0, s, acc, n, base, base + m);
         hipLaunchKernelGGL(solve_k, dim3((unsigned)((m + solve_w - 1) / solve_w)), dim3(64 * solve_w), 0, s, acc, n,
                            fail_first, ordered, base, base + m);
         hipLaunchKernelGGL((wv::k_sign_retry<TableSign, KL>), dim3((unsigned)(m < 256 ? m : 256)), dim3(retry_threads), 0, s,
@@ -568,9 +575,10 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
             const int fail_first = (stages >> 8) & 0xFF;
-            // shared-key tables (prepared keys, split launches): items in
-            // key order, several per workgroup
-            const bool shared = std::is_same<KL, wv::PrepKeys>::value && cfg.sign_chunks != 0 && wv::sk_code(cfg.prep);
+            // shared-key tables (prepared keys, split launches, a T region in
+            // the workspace): items in key order, several per workgroup
+            const bool shared = std::is_same<KL, wv::PrepKeys>::value && cfg.sign_chunks != 0 &&
+                                wv::sk_code(cfg.prep) && acc.trows;
             const int ordered = shared || (cfg.sign_order && n > 64);
             const int parts = cfg.sign_chunks > 1 && n >= CHUNK_MIN ? 2 : 1;
             if (shared)
@@ -581,6 +589,7 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
                 // one persistent fused launch (prepared or reference tables)
                 bool found = false;
                 const bool own = std::is_same<KL, wv::PrepKeys>::value && !wv::sk_code(cfg.prep);
+                // (the shared-key codes stand for PREP_OCCS[0] here)
                 with_occ<wv::PREP_OCCS>(own ? cfg.prep : wv::PREP_OCCS[0], [&](auto P) {
                     constexpr int code = decltype(P)::value;
                     found = true;
@@ -600,19 +609,20 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
                     with_occ<wv::SK_CODES>(cfg.prep, [&](auto P) {
                         with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto Q) {
                             found = true;
-                            e = launch_sign_stage2<KL>(&wv::k_sign_prep_shared<TableSign, decltype(P)::value>,
-                                                       wv::sk_w(decltype(P)::value),
+                            e = launch_sign_stage2<KL>(&wv::k_sign_table_shared<TableSign, decltype(P)::value>,
+                                                       wv::sk_w(decltype(P)::value), &wv::k_sign_fold<TableSign>,
                                                        &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
                                                        wv::solve_sigs(decltype(Q)::value), acc, n, ordered,
                                                        fail_first, parts, stream);
                         });
                     });
                 } else {
-                    with_occ<wv::PREP_OCCS>(cfg.prep, [&](auto P) {
+                    // a shared-key code without a T region runs the wave kernel
+                    with_occ<wv::PREP_OCCS>(wv::sk_code(cfg.prep) ? wv::PREP_OCCS[0] : cfg.prep, [&](auto P) {
                         with_occ<wv::SOLVE_OCCS>(cfg.solve, [&](auto Q) {
                             found = true;
                             e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL, decltype(P)::value>, 1,
-                                                       &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
+                                                       nullptr, &wv::k_sign_solve<TableSign, KL::PLANAR, decltype(Q)::value>,
                                                        wv::solve_sigs(decltype(Q)::value), acc, n, ordered,
                                                        fail_first, parts, stream);
                         });
@@ -623,7 +633,7 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
                     return (int)hipErrorInvalidValue;
                 }
             } else {
-                e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL>, 1,
+                e = launch_sign_stage2<KL>(&wv::k_sign_prep<TableSign, KL>, 1, nullptr,
                                            &wv::k_sign_solve<TableSign, KL::PLANAR>, 1, acc, n, ordered, fail_first,
                                            parts, stream);
             }
@@ -650,7 +660,7 @@ int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int 
         if (stages & 4) {
             k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
-            const bool shared = std::is_same<KL, wv::PrepKeys>::value && wv::sk_code(cfg.verify);
+            const bool shared = std::is_same<KL, wv::PrepKeys>::value && wv::sk_code(cfg.verify) && acc.trows;
             const int ordered = !shared && cfg.verify_order && n > 64;
             if (shared)
                 wv::k_key_sort<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
@@ -659,9 +669,12 @@ int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int 
             if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
                 const bool found = shared ? with_occ<wv::SK_CODES>(cfg.verify, [&](auto O) {
                     constexpr int W = wv::sk_w(decltype(O)::value);
-                    wv::k_verify_shared<TableVerify, decltype(O)::value>
+                    wv::k_verify_table_shared<TableVerify, decltype(O)::value>
                         <<<(unsigned)((n + W - 1) / W), 64 * W, 0, stream>>>(acc, n);
-                }) : with_occ<wv::VERIFY_OCCS>(cfg.verify, [&](auto O) {
+                    wv::k_verify_fold<TableVerify>
+                        <<<(unsigned)((n + wv::FOLD_W - 1) / wv::FOLD_W), 64 * wv::FOLD_W, 0, stream>>>(acc, n);
+                }) : with_occ<wv::VERIFY_OCCS>(wv::sk_code(cfg.verify) ? wv::VERIFY_OCCS[0] : cfg.verify,
+                                               [&](auto O) {
                     wv::k_verify_wave<TableVerify, KL, decltype(O)::value>
                         <<<(unsigned)n, 64, 0, stream>>>(acc, n, ordered);
                 });
@@ -727,16 +740,31 @@ int cumayo_prepare_epk(const uint8_t *epk_table, size_t epk_stride, size_t n_key
                         stream);
 }
 
-int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
-                         const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
-                         uint8_t *workspace, hipStream_t stream, int stages) {
+}  // extern "C"
+
+namespace {
+// cumayo_sign_prepared over per-item slots at `workspace` and T regions at
+// `trows` (none: the wave kernels run whatever the launch configuration)
+int sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs, const int64_t *msg_off,
+                  const int32_t *msg_len, uint8_t *sigs, size_t n, uint8_t *workspace, uint8_t *trows,
+                  hipStream_t stream, int stages) {
     if (!n) return 0;
     if (!batch_ok(n)) return (int)hipErrorInvalidValue;
     if (!prepared || ((uintptr_t)prepared & 15) || !key_idx || !msgs || !msg_off || !msg_len || !sigs ||
         !workspace || (stages & 4))
         return (int)hipErrorInvalidValue;
-    TableSign acc{prepared, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, workspace};
+    TableSign acc{prepared, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, workspace, trows};
     return sign_table_impl<wv::PrepKeys>(acc, n, stream, stages, current_cfg());
+}
+}  // namespace
+
+extern "C" {
+
+int cumayo_sign_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
+                         const int64_t *msg_off, const int32_t *msg_len, uint8_t *sigs, size_t n,
+                         uint8_t *workspace, hipStream_t stream, int stages) {
+    uint8_t *trows = workspace ? workspace + n * (size_t)SIGN_SCRATCH_STRIDE : nullptr;
+    return sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, workspace, trows, stream, stages);
 }
 
 // cumayo_sign_prepared with its workspace in `scratch`, a file-scope device
@@ -788,9 +816,9 @@ int cumayo_sign_prepared_noalloc(const uint8_t *prepared, const int32_t *key_idx
     uint8_t *ws = scratch_ptr();
     if (!ws) return (int)hipErrorInvalidSymbol;
     // one launch part (n is far below the two-part threshold): no helper
-    // stream or event is created
+    // stream or event is created; no T regions, so the wave kernels run
     static_assert(CUMAYO_NOALLOC_MAX_SIGN < CHUNK_MIN, "the no-allocation path runs stage 2 in one part");
-    return cumayo_sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, ws, stream, 3);
+    return sign_prepared(prepared, key_idx, msgs, msg_off, msg_len, sigs, n, ws, nullptr, stream, 3);
 }
 
 int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, const uint8_t *msgs,
@@ -803,7 +831,7 @@ int cumayo_verify_prepared(const uint8_t *prepared, const int32_t *key_idx, cons
         !results || !workspace || (stages & 4))
         return (int)hipErrorInvalidValue;
     TableVerify acc{prepared, wv::PrepKeys::EPK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, results,
-                    workspace};
+                    workspace, workspace + n * (size_t)VERIFY_SCRATCH_STRIDE};
     return verify_table_impl<wv::PrepKeys>(acc, n, stream, stages, current_cfg());
 }
 
@@ -814,7 +842,7 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
                                     workspace, stream, 3);
 }
 
-size_t cumayo_verify_workspace_size(size_t n) { return n * (size_t)VERIFY_SCRATCH_STRIDE; }
+size_t cumayo_verify_workspace_size(size_t n) { return n * ((size_t)VERIFY_SCRATCH_STRIDE + VERIFY_T_BYTES); }
 
 int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
                                const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
@@ -1015,7 +1043,8 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
     LaunchCfg cfg = current_cfg();
     int err = 0, timed = 0;
     if (do_sign) {
-        TableSign acc{prep_esk, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, sign_ws};
+        TableSign acc{prep_esk, wv::PrepKeys::ESK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, sign_ws,
+                      sign_ws + n * (size_t)SIGN_SCRATCH_STRIDE};
         err = sign_table_impl<wv::PrepKeys>(acc, n, stream, 1, cfg);  // hashing pass, once
         double best = 1e30;
         LaunchCfg bc = cfg;
@@ -1046,7 +1075,7 @@ int cumayo_autotune(const uint8_t *prep_esk, const uint8_t *prep_epk, const int3
     }
     if (do_verify && !err) {
         TableVerify acc{prep_epk, wv::PrepKeys::EPK_BYTES, key_idx, msgs, msg_off, msg_len, sigs, results,
-                        verify_ws};
+                        verify_ws, verify_ws + n * (size_t)VERIFY_SCRATCH_STRIDE};
         err = verify_table_impl<wv::PrepKeys>(acc, n, stream, 1, cfg);
         double best = 1e30;
         LaunchCfg bc = cfg;

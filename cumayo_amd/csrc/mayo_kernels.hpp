@@ -47,6 +47,12 @@ constexpr uint8_t ATTEMPT_PENDING = 0xFF;   // rank failure left for the retry p
 // Scratch per verifying item: t, and the hashing order slot.
 constexpr int VSCR_ORDER = (MB + 3) / 4 * 4;
 constexpr int VERIFY_SCRATCH_STRIDE = (VSCR_ORDER + 4 + 15) / 16 * 16;
+// Table-form workspaces end with one T region per item for the shared-key
+// variants: T_j[r] of every row r the pair sums fold (the V rows of P1 for
+// signing, the N rows of [[P1, P2], [0, P3]] for verification), K m-vectors
+// of MW words per row, row-major (r K + j) MW + w, planar words.
+constexpr size_t SIGN_T_BYTES = (size_t)V * K * MW * 4;
+constexpr size_t VERIFY_T_BYTES = (size_t)N * K * MW * 4;
 // Hashing order: items grouped by the SHAKE256 blocks (136 bytes) their
 // message absorbs; ORDER_BINS - 1 and more share the last group.
 constexpr int ORDER_BINS = 256;
@@ -90,6 +96,7 @@ struct TableSign {
     const int32_t *msg_len;
     uint8_t *sigs;
     uint8_t *scratch;
+    uint8_t *trows = nullptr;  // T regions (SIGN_T_BYTES per item), or none
     __device__ SignItem operator()(size_t i) const {
         return {esk_table + (size_t)key_idx[i] * esk_stride, msgs + msg_off[i],
                 (uint32_t)msg_len[i], sigs + i * (size_t)SIG_BYTES,
@@ -99,6 +106,7 @@ struct TableSign {
     __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
     __device__ uint32_t key(size_t i) const { return (uint32_t)key_idx[i]; }
     __device__ const uint8_t *key_row(uint32_t k) const { return esk_table + (size_t)k * esk_stride; }
+    __device__ uint32_t *trow(size_t i) const { return (uint32_t *)(trows + i * SIGN_T_BYTES); }
     __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + SCR_ORDER) = i; }
     __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + SCR_ORDER); }
 };
@@ -133,6 +141,7 @@ struct TableVerify {
     const uint8_t *sigs;
     uint8_t *results;
     uint8_t *scratch;
+    uint8_t *trows = nullptr;  // T regions (VERIFY_T_BYTES per item), or none
     __device__ VerifyItem operator()(size_t i) const {
         return {epk_table + (size_t)key_idx[i] * epk_stride, msgs + msg_off[i],
                 (uint32_t)msg_len[i], sigs + i * (size_t)SIG_BYTES, results + i,
@@ -142,6 +151,7 @@ struct TableVerify {
     __device__ uint32_t bin(size_t i) const { return order_bin(msg_len[i]); }
     __device__ uint32_t key(size_t i) const { return (uint32_t)key_idx[i]; }
     __device__ const uint8_t *key_row(uint32_t k) const { return epk_table + (size_t)k * epk_stride; }
+    __device__ uint32_t *trow(size_t i) const { return (uint32_t *)(trows + i * VERIFY_T_BYTES); }
     __device__ void set_order(size_t k, uint32_t i) const { *(uint32_t *)(scratch_of(k) + VSCR_ORDER) = i; }
     __device__ size_t order(size_t k) const { return *(const uint32_t *)(scratch_of(k) + VSCR_ORDER); }
 };

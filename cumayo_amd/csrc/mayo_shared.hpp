@@ -115,16 +115,9 @@ constexpr bool sk_code(int code) { return code >= 1000; }
 constexpr int sk_w(int code) { return (code % 1000) / 10; }
 constexpr int sk_cg(int code) { return code % 10; }
 
-// LDS of one wavefront
+// LDS of one wavefront of a table kernel
 struct SkWave {
     uint64_t scol[N];  // scol[c] nibble j = S_j[c]
-    union {
-        uint32_t T[SK_RPC * K * MW];  // the chunk's rows: T_j of row slot at (slot K + j) MW (planar)
-        struct {
-            uint32_t U[PAIRS * MW];     // pair sums, nibble-packed
-            uint32_t yu[YG_MAX * UW];   // unreduced y
-        } y;
-    };
 };
 
 template <int W, int CG>
@@ -134,17 +127,17 @@ struct SkGroup {
     int key[W];  // key index of each wavefront's item, -1 for none
 };
 
-// Quadratic form of one key for the wavefronts with `mine` set, built by the
-// whole workgroup (every wavefront calls it; it ends with a workgroup
-// barrier). Rows: with NLR > 0 (signing), the columns x of L (entries
-// L[c][x], every c) and then the rows of P1; with FULL (verification), the
-// rows of [[P1, P2], [0, P3]] over the columns c < n. Entries come from a
-// prepared key row (aligned, planar words). U[i][a][h] accumulates
-// S_a[r] T_j[r] (word 2 gp + h) of fold item i = (j, gp); the rows of L
-// store M_j[x] to Mv (global, planar, (j O + x) MW + w).
+// The column loop of one key's quadratic form for the wavefronts with `mine`
+// set, the tables built by the whole workgroup (every wavefront calls it; it
+// ends with a workgroup barrier). Rows: with NLR > 0 (signing), the columns x
+// of L (entries L[c][x], every c) and then the rows of P1; with FULL
+// (verification), the rows of [[P1, P2], [0, P3]] over the columns c < n.
+// Entries come from a prepared key row (aligned, planar words). The rows of
+// L store M_j[x] to Mv (planar, (j O + x) MW + w), the P rows T_j[r] to Tg
+// (planar, (r K + j) MW + w).
 template <int NLR, bool FULL, int W, int CG>
-__device__ __forceinline__ void quad_form_shared(const uint8_t *key, bool mine, SkGroup<W, CG> &G, SkWave &A,
-                                                 uint32_t (&U)[SK_IPL][K][SK_MWL], uint32_t *Mv) {
+__device__ __forceinline__ void table_rows_shared(const uint8_t *key, bool mine, SkGroup<W, CG> &G, SkWave &A,
+                                                  uint32_t *Mv, uint32_t *Tg) {
     using KL = PrepKeys;
     constexpr int EB = KL::EB;
     constexpr int NP = FULL ? N : V;   // P rows and columns
@@ -292,75 +285,22 @@ __device__ __forceinline__ void quad_form_shared(const uint8_t *key, bool mine, 
             T[j][1] ^= v[j].y;
         }
     };
-    // The fold of one row (row `row` of the chunk starting at row v0 of the
-    // schedule) into U, in two steps so that the table reads of a column can
-    // be in flight between them: fold_load issues the row's LDS reads (its
-    // scalars and the items' T words), fold_compute does the products.
-    struct FoldIn {
-        uint64_t sr;
-        uint2 tv[SK_IPL];
-    };
-    auto fold_load = [&](int v0, int row) {
-        FoldIn f;
-        f.sr = A.scol[v0 + row - NLR];
-#pragma unroll
-        for (int i = 0; i < SK_IPL; ++i) {
-            const int it = lane + 64 * i;
-            const int j = it / SK_NWP, gp = it % SK_NWP;
-            f.tv[i] = it < SK_NFI ? *(const uint2 *)&A.T[(row * K + j) * MW + 2 * gp] : uint2{0u, 0u};
-        }
-        return f;
-    };
-    auto fold_compute = [&](const FoldIn &f) {
-        const uint64_t sr = uni64(f.sr);
-        const uint32_t sh[2] = {(uint32_t)sr, (uint32_t)(sr >> 32)};
-#pragma unroll
-        for (int i = 0; i < SK_IPL; ++i) {
-            if (lane + 64 * i >= SK_NFI) continue;
-            const gf16::planar::Multiples m0(f.tv[i].x), m1(f.tv[i].y);
-#pragma unroll
-            for (int a = 0; a < K; ++a) {
-                const int bit = 4 * a;
-                const uint32_t mk0 = gf16::bitmask(sh[bit >> 5], bit & 31);
-                const uint32_t mk1 = gf16::bitmask(sh[(bit + 1) >> 5], (bit + 1) & 31);
-                const uint32_t mk2 = gf16::bitmask(sh[(bit + 2) >> 5], (bit + 2) & 31);
-                const uint32_t mk3 = gf16::bitmask(sh[(bit + 3) >> 5], (bit + 3) & 31);
-                U[i][a][0] = gf16::mac(U[i][a][0], m0.x0, m0.x1, m0.x2, m0.x3, mk0, mk1, mk2, mk3);
-                U[i][a][1] = gf16::mac(U[i][a][1], m1.x0, m1.x1, m1.x2, m1.x3, mk0, mk1, mk2, mk3);
-            }
-        }
-    };
-    auto fold_row = [&](int v0, int row) { fold_compute(fold_load(v0, row)); };
-    // The fold of a chunk's P rows runs during the next chunk's intervals,
-    // a share of the rows in each, so that its VALU work interleaves with
-    // the table reads; rows [f_row, f_end) of the chunk at f_v0 are pending.
-    int f_v0 = 0, f_row = 0, f_end = 0;
-    auto fold_some = [&](int nrows) {
-#pragma unroll 1
-        for (int k = 0; k < nrows && f_row < f_end; ++k, ++f_row) fold_row(f_v0, f_row);
-    };
-    // end of chunk ch: the previous chunk's fold completes, then T goes out
-    // (L rows to Mv, P rows to A.T) and the chunk's P rows become pending
+    // end of chunk ch: the L rows go to Mv, the P rows to Tg
     auto chunk_end = [&](int ch) {
-        fold_some(SK_RPC);
-        WaveSync::sync();  // the fold's reads of A.T before the stores below
-        const int v0 = ch * SK_RPC, vr = v0 + slot;
+        const int vr = ch * SK_RPC + slot;
         if (active && vr < NVR) {
             if (vr < NLR) {
 #pragma unroll
                 for (int j = 0; j < K; ++j)
                     *(uint2 *)&Mv[(j * O + vr) * MW + 2 * g] = uint2{T[j][0], T[j][1]};
             } else {
+                uint32_t *row = Tg + (size_t)(vr - NLR) * K * MW + 2 * g;
 #pragma unroll
-                for (int j = 0; j < K; ++j) *(uint2 *)&A.T[(slot * K + j) * MW + 2 * g] = uint2{T[j][0], T[j][1]};
+                for (int j = 0; j < K; ++j) *(uint2 *)&row[j * MW] = uint2{T[j][0], T[j][1]};
             }
         }
 #pragma unroll
         for (int j = 0; j < K; ++j) T[j][0] = T[j][1] = 0;
-        WaveSync::sync();
-        f_v0 = v0;
-        f_row = (v0 > NLR ? v0 : NLR) - v0;
-        f_end = (v0 + SK_RPC < NVR ? v0 + SK_RPC : NVR) - v0;
     };
     // the interval sequence: (chunk, first column), consumed in order; the
     // table of interval i + 1 is produced during interval i
@@ -394,32 +334,16 @@ __device__ __forceinline__ void quad_form_shared(const uint8_t *key, bool mine, 
     for (;;) {
         int nch = ch, nc0 = c0;
         next(nch, nc0);
-        // Per interval: the first fold row's loads and column 0's table
-        // reads go out first; the next table's production and the fold
-        // row's products run while they are in flight.
+        // column 0's table reads go out first; the next table's production
+        // runs while they are in flight
         uint2 v[K];
-        FoldIn fin;
-        int nf = 0;
-        if (mine) {
-            if (nch == ch && f_row < f_end) {
-                // pending fold rows over the intervals left in this chunk
-                const int left = (NP - c0 + CG - 1) / CG;
-                nf = (f_end - f_row + left - 1) / left;
-                fin = fold_load(f_v0, f_row);
-            }
-            issue(buf, 0, v);
-        }
+        if (mine) issue(buf, 0, v);
         if (pch < NCH) {
             produce(buf ^ 1, xr[0]);
             advance();
         }
         if (mine) {
-            if (nf > 0) {
-                fold_compute(fin);
-                ++f_row;
-            }
             absorb(v);
-            if (nf > 1) fold_some(nf - 1);
 #pragma unroll
             for (int cl = 1; cl < CG; ++cl) {
                 if (c0 + cl >= NP) break;
@@ -435,29 +359,6 @@ __device__ __forceinline__ void quad_form_shared(const uint8_t *key, bool mine, 
         if (ch >= NCH) break;
         buf ^= 1;
     }
-    if (mine) fold_some(SK_RPC);  // the last chunk's rows
-}
-
-// The pair sums of the fold items' ordered products into A.y.U (nibble
-// words): U_(a,j) + U_(j,a) for a < j, U_(a,a).
-__device__ __forceinline__ void sk_pair_sums(SkWave &A, const uint32_t (&U)[SK_IPL][K][SK_MWL]) {
-    const int lane = lane_here();
-    for (int e = lane; e < PAIRS * MW; e += 64) A.y.U[e] = 0;
-    WaveSync::sync();
-#pragma unroll
-    for (int i = 0; i < SK_IPL; ++i) {
-        const int it = lane + 64 * i;
-        if (it >= SK_NFI) continue;
-        const int j = it / SK_NWP, gp = it % SK_NWP;
-#pragma unroll
-        for (int a = 0; a < K; ++a) {
-            const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
-#pragma unroll
-            for (int h = 0; h < SK_MWL; ++h)
-                atomicXor(&A.y.U[p * MW + 2 * gp + h], gf16::planar::to_nibbles(U[i][a][h]));
-        }
-    }
-    WaveSync::sync();
 }
 
 // The workgroup's items and their keys; kernels then loop over the keys.
@@ -485,12 +386,12 @@ __device__ __forceinline__ int sk_next_key(const SkGroup<W, CG> &G, int done, in
     return sel;
 }
 
-// Batch form over prepared keys, attempt 0, part 1 (as k_sign_prep): M, U
-// and y of positions [base, end) of the key order, W per workgroup; the
-// failure list of the part starts empty.
+// Batch form over prepared keys, attempt 0, part 1 (as k_sign_prep, with
+// k_sign_fold after it): M and the T rows of positions [base, end) of the
+// key order, W per workgroup; the failure list of the part starts empty.
 template <class Acc, int CODE>
-__global__ void __launch_bounds__(64 * sk_w(CODE), 4)
-    k_sign_prep_shared(Acc acc, size_t n, int ordered, size_t base, size_t end) {
+__global__ void __launch_bounds__(64 * sk_w(CODE), 8)
+    k_sign_table_shared(Acc acc, size_t n, int ordered, size_t base, size_t end) {
     constexpr int W = sk_w(CODE), CG = sk_cg(CODE);
     __shared__ SkGroup<W, CG> G;
     const int tid = threadIdx.x, lane = tid & 63, wave = (int)uni(threadIdx.x >> 6);
@@ -512,39 +413,23 @@ __global__ void __launch_bounds__(64 * sk_w(CODE), 4)
         ((uint32_t *)G.tab[1])[q] = 0;
     }
     __syncthreads();
-    uint32_t U[SK_IPL][K][SK_MWL];
-#pragma unroll
-    for (int i = 0; i < SK_IPL; ++i)
-#pragma unroll
-        for (int a = 0; a < K; ++a) U[i][a][0] = U[i][a][1] = 0;
     uint32_t *Mv = has ? (uint32_t *)(it.scratch + SCR_MV) : nullptr;
+    uint32_t *Tg = has ? acc.trow(item) : nullptr;
     int done = 0;
 #pragma unroll 1
     for (;;) {
         int kk;
         const int sel = sk_next_key(G, done, kk);
         if (kk < 0) break;
-        quad_form_shared<O, false, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, U, Mv);
+        table_rows_shared<O, false, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, Mv, Tg);
         done |= sel;
-    }
-    if (!has) return;  // whole wavefronts; no workgroup barrier follows
-    sk_pair_sums(A, U);
-    unreduced_y<WaveSync>(A.y.U, A.y.yu);
-    if (lane == 0) {
-        uint32_t a[UW], out[MW];
-#pragma unroll
-        for (int q = 0; q < UW; ++q) a[q] = A.y.yu[q];
-        reduce_unreduced(a, out);
-        uint32_t *yv = (uint32_t *)(it.scratch + SCR_Y);
-#pragma unroll
-        for (int q = 0; q < MW; ++q) yv[q] = out[q] ^ mvec::load_mw(it.scratch + SCR_T, q);
     }
 }
 
-// Verification over prepared keys (as k_verify_wave, t from the hashing
-// pass), positions [0, n) of the key order, W per workgroup.
+// Verification over prepared keys, part 1 (k_verify_fold after it): the T
+// rows of positions [0, n) of the key order, W per workgroup.
 template <class Acc, int CODE>
-__global__ void __launch_bounds__(64 * sk_w(CODE), 4) k_verify_shared(Acc acc, size_t n) {
+__global__ void __launch_bounds__(64 * sk_w(CODE), 8) k_verify_table_shared(Acc acc, size_t n) {
     constexpr int W = sk_w(CODE), CG = sk_cg(CODE);
     __shared__ SkGroup<W, CG> G;
     const int tid = threadIdx.x, lane = tid & 63, wave = (int)uni(threadIdx.x >> 6);
@@ -564,27 +449,167 @@ __global__ void __launch_bounds__(64 * sk_w(CODE), 4) k_verify_shared(Acc acc, s
         ((uint32_t *)G.tab[1])[q] = 0;
     }
     __syncthreads();
-    uint32_t U[SK_IPL][K][SK_MWL];
-#pragma unroll
-    for (int i = 0; i < SK_IPL; ++i)
-#pragma unroll
-        for (int a = 0; a < K; ++a) U[i][a][0] = U[i][a][1] = 0;
+    uint32_t *Tg = has ? acc.trow(item) : nullptr;
     int done = 0;
 #pragma unroll 1
     for (;;) {
         int kk;
         const int sel = sk_next_key(G, done, kk);
         if (kk < 0) break;
-        quad_form_shared<0, true, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, U, nullptr);
+        table_rows_shared<0, true, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, nullptr, Tg);
         done |= sel;
     }
-    if (!has) return;
-    sk_pair_sums(A, U);
-    unreduced_y<WaveSync>(A.y.U, A.y.yu);
+}
+
+// ---- the fold -------------------------------------------------------------------
+// U_(a,j) += S_a[r] T_j[r] over the NP rows of T (one wavefront per item), in
+// the transposed layout: lanes own fold items (j, word pair gp) and keep
+// U_(a,j) for every a in registers (2K per item); a row's scalars S_a[r] are
+// uniform, so their masks are SGPR masks, and the lanes' T words of a row are
+// K MW consecutive words. Rows are read FOLD_AHEAD ahead of their use.
+// U_(a,j) + U_(j,a) is the pair sum of the bitsliced kernels
+// (sign.cu:395-437, verify.cu:72-182).
+constexpr int FOLD_AHEAD = 4;
+constexpr int FOLD_W = 4;  // wavefronts per workgroup
+
+struct SkFold {
+    uint64_t scol[N];
+    uint32_t U[PAIRS * MW];     // pair sums, nibble-packed
+    uint32_t yu[YG_MAX * UW];   // unreduced y
+};
+
+template <int NP>
+__device__ __forceinline__ void fold_rows(const uint64_t *scol, const uint32_t *Tg, uint32_t (&U)[SK_IPL][K][SK_MWL]) {
+    const int lane = lane_here();
+    constexpr int RW = K * MW;  // words of a row
+    constexpr int NR = (NP + FOLD_AHEAD - 1) / FOLD_AHEAD * FOLD_AHEAD;
+#pragma unroll
+    for (int i = 0; i < SK_IPL; ++i)
+#pragma unroll
+        for (int a = 0; a < K; ++a) U[i][a][0] = U[i][a][1] = 0;
+    // word pair of fold item it = (j, gp) in row r: r K MW + j MW + 2 gp = r RW + 2 it
+    int off[SK_IPL];
+#pragma unroll
+    for (int i = 0; i < SK_IPL; ++i) {
+        const int it = lane + 64 * i;
+        off[i] = it < SK_NFI ? 2 * it : 0;
+    }
+    auto load = [&](int r, uint2 (&t)[SK_IPL]) {
+        const uint32_t *row = Tg + (size_t)(r < NP ? r : NP - 1) * RW;
+#pragma unroll
+        for (int i = 0; i < SK_IPL; ++i) t[i] = *(const uint2 *)(row + off[i]);
+    };
+    uint2 ring[FOLD_AHEAD][SK_IPL];
+#pragma unroll
+    for (int d = 0; d < FOLD_AHEAD; ++d) load(d, ring[d]);
+#pragma unroll 1
+    for (int r0 = 0; r0 < NR; r0 += FOLD_AHEAD) {
+#pragma unroll
+        for (int d = 0; d < FOLD_AHEAD; ++d) {
+            const int r = r0 + d;
+            // rows past the last read the last row again and add nothing
+            const uint64_t sr = uni64(r < NP ? scol[r] : 0ull);
+            const uint32_t sh[2] = {(uint32_t)sr, (uint32_t)(sr >> 32)};
+#pragma unroll
+            for (int i = 0; i < SK_IPL; ++i) {
+                const gf16::planar::Multiples m0(ring[d][i].x), m1(ring[d][i].y);
+#pragma unroll
+                for (int a = 0; a < K; ++a) {
+                    const int bit = 4 * a;
+                    const uint32_t mk0 = gf16::bitmask(sh[bit >> 5], bit & 31);
+                    const uint32_t mk1 = gf16::bitmask(sh[(bit + 1) >> 5], (bit + 1) & 31);
+                    const uint32_t mk2 = gf16::bitmask(sh[(bit + 2) >> 5], (bit + 2) & 31);
+                    const uint32_t mk3 = gf16::bitmask(sh[(bit + 3) >> 5], (bit + 3) & 31);
+                    U[i][a][0] = gf16::mac(U[i][a][0], m0.x0, m0.x1, m0.x2, m0.x3, mk0, mk1, mk2, mk3);
+                    U[i][a][1] = gf16::mac(U[i][a][1], m1.x0, m1.x1, m1.x2, m1.x3, mk0, mk1, mk2, mk3);
+                }
+            }
+            load(r + FOLD_AHEAD, ring[d]);
+        }
+    }
+}
+
+// The pair sums of the fold items' ordered products into Ul (nibble words):
+// U_(a,j) + U_(j,a) for a < j, U_(a,a). Lanes of fold items past the last
+// hold zeros and add nothing.
+__device__ __forceinline__ void sk_pair_sums(uint32_t *Ul, const uint32_t (&U)[SK_IPL][K][SK_MWL]) {
+    const int lane = lane_here();
+    for (int e = lane; e < PAIRS * MW; e += 64) Ul[e] = 0;
+    WaveSync::sync();
+#pragma unroll
+    for (int i = 0; i < SK_IPL; ++i) {
+        const int it = lane + 64 * i;
+        if (it >= SK_NFI) continue;
+        const int j = it / SK_NWP, gp = it % SK_NWP;
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+            const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
+#pragma unroll
+            for (int h = 0; h < SK_MWL; ++h)
+                atomicXor(&Ul[p * MW + 2 * gp + h], gf16::planar::to_nibbles(U[i][a][h]));
+        }
+    }
+    WaveSync::sync();
+}
+
+// Signing, attempt 0, part 1 after k_sign_table_shared: U and y of positions
+// [base, end) of the key order, one wavefront per item.
+template <class Acc>
+__global__ void __launch_bounds__(64 * FOLD_W) k_sign_fold(Acc acc, size_t n, size_t base, size_t end) {
+    __shared__ SkFold F[FOLD_W];
+    const int lane = lane_here(), wave = (int)uni(threadIdx.x >> 6);
+    const size_t k = base + (size_t)blockIdx.x * FOLD_W + wave;
+    if (k >= end) return;  // whole wavefronts; no workgroup barrier follows
+    const size_t item = acc.order(k);
+    SkFold &A = F[wave];
+    const SignItem it = acc(item);
+    const uint8_t *vr = it.scratch + SCR_VR;
+    for (int c = lane; c < V; c += 64) {
+        uint64_t v = 0;
+        for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(vr + j * V_BYTES, c) << (4 * j);
+        A.scol[c] = v;
+    }
+    WaveSync::sync();
+    uint32_t U[SK_IPL][K][SK_MWL];
+    fold_rows<V>(A.scol, acc.trow(item), U);
+    sk_pair_sums(A.U, U);
+    unreduced_y<WaveSync>(A.U, A.yu);
+    if (lane == 0) {
+        uint32_t a[UW], out[MW];
+#pragma unroll
+        for (int q = 0; q < UW; ++q) a[q] = A.yu[q];
+        reduce_unreduced(a, out);
+        uint32_t *yv = (uint32_t *)(it.scratch + SCR_Y);
+#pragma unroll
+        for (int q = 0; q < MW; ++q) yv[q] = out[q] ^ mvec::load_mw(it.scratch + SCR_T, q);
+    }
+}
+
+// Verification after k_verify_table_shared: y of positions [0, n) of the key
+// order against t (from the hashing pass), one wavefront per item.
+template <class Acc>
+__global__ void __launch_bounds__(64 * FOLD_W) k_verify_fold(Acc acc, size_t n) {
+    __shared__ SkFold F[FOLD_W];
+    const int lane = lane_here(), wave = (int)uni(threadIdx.x >> 6);
+    const size_t k = (size_t)blockIdx.x * FOLD_W + wave;
+    if (k >= n) return;
+    const size_t item = acc.order(k);
+    SkFold &A = F[wave];
+    const VerifyItem it = acc(item);
+    for (int c = lane; c < N; c += 64) {
+        uint64_t v = 0;
+        for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(it.sig, j * N + c) << (4 * j);
+        A.scol[c] = v;
+    }
+    WaveSync::sync();
+    uint32_t U[SK_IPL][K][SK_MWL];
+    fold_rows<N>(A.scol, acc.trow(item), U);
+    sk_pair_sums(A.U, U);
+    unreduced_y<WaveSync>(A.U, A.yu);
     if (lane == 0) {
         uint32_t a[UW], y[MW];
 #pragma unroll
-        for (int q = 0; q < UW; ++q) a[q] = A.y.yu[q];
+        for (int q = 0; q < UW; ++q) a[q] = A.yu[q];
         reduce_unreduced(a, y);
         bool eq = true;
 #pragma unroll

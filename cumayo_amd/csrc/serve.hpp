@@ -208,8 +208,8 @@ class Server {
             dev_alloc(&s.d_sig, mb * mayo::SIG_BYTES);
             dev_alloc(&s.d_res, mb);
             dev_alloc(&s.d_att, mb);
-            dev_alloc(&s.d_wss, mb * (size_t)mayo::SIGN_SCRATCH_STRIDE);
-            dev_alloc(&s.d_wsv, mb * (size_t)mayo::VERIFY_SCRATCH_STRIDE);
+            dev_alloc(&s.d_wss, cumayo_sign_workspace_size(mb));
+            dev_alloc(&s.d_wsv, cumayo_verify_workspace_size(mb));
             s.reqs.reserve(mb);
             s.vreqs.reserve(mb);
             s.tmp.reserve(mb);
