@@ -367,7 +367,7 @@ __device__ __forceinline__ size_t sk_item(const Acc &acc, SkGroup<W, CG> &G, siz
     const int wave = (int)uni(threadIdx.x >> 6);
     const size_t k = base + (size_t)blockIdx.x * W + wave;
     has = k < end;
-    const size_t item = has ? acc.order(k) : 0;
+    const size_t item = has ? uni64(acc.order(k)) : 0;  // uniform: scalar registers
     if ((threadIdx.x & 63) == 0) G.key[wave] = has ? (int)acc.key(item) : -1;
     return item;
 }
@@ -421,7 +421,7 @@ __global__ void __launch_bounds__(64 * sk_w(CODE), 8)
         int kk;
         const int sel = sk_next_key(G, done, kk);
         if (kk < 0) break;
-        table_rows_shared<O, false, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, Mv, Tg);
+        table_rows_shared<O, false, W, CG>(acc.key_row(uni((uint32_t)kk)), (sel >> wave) & 1, G, A, Mv, Tg);
         done |= sel;
     }
 }
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(64 * sk_w(CODE), 8) k_verify_table_shared(Acc 
         int kk;
         const int sel = sk_next_key(G, done, kk);
         if (kk < 0) break;
-        table_rows_shared<0, true, W, CG>(acc.key_row((uint32_t)kk), (sel >> wave) & 1, G, A, nullptr, Tg);
+        table_rows_shared<0, true, W, CG>(acc.key_row(uni((uint32_t)kk)), (sel >> wave) & 1, G, A, nullptr, Tg);
         done |= sel;
     }
 }
@@ -469,13 +469,36 @@ __global__ void __launch_bounds__(64 * sk_w(CODE), 8) k_verify_table_shared(Acc 
 // K MW consecutive words. Rows are read FOLD_AHEAD ahead of their use.
 // U_(a,j) + U_(j,a) is the pair sum of the bitsliced kernels
 // (sign.cu:395-437, verify.cu:72-182).
-constexpr int FOLD_AHEAD = 4;
+constexpr int FOLD_AHEAD = 8;
 constexpr int FOLD_W = 4;  // wavefronts per workgroup
 
 struct SkFold {
     uint64_t scol[N];
+    uint32_t t[MW];             // t, read before the fold
     uint32_t U[PAIRS * MW];     // pair sums, nibble-packed
     uint32_t yu[YG_MAX * UW];   // unreduced y
+};
+
+// s x for the 16 scalars s (bit b of s is the coefficient of a^b), held in
+// registers; an index uniform across the wavefront selects one without
+// leaving them
+struct SixteenMultiples {
+    typedef uint32_t v16 __attribute__((ext_vector_type(16)));
+    v16 m;
+    __device__ __forceinline__ explicit SixteenMultiples(uint32_t x) {
+        using gf16::planar::mul_a;
+        const uint32_t x2 = mul_a(x), x4 = mul_a(x2), x8 = mul_a(x4);
+        m[0] = 0u;
+        m[1] = x;
+        m[2] = x2;
+        m[3] = x ^ x2;
+        m[4] = x4;
+        m[5] = x ^ x4;
+        m[6] = x2 ^ x4;
+        m[7] = m[3] ^ x4;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) m[8 + q] = m[q] ^ x8;
+    }
 };
 
 template <int NP>
@@ -512,16 +535,17 @@ __device__ __forceinline__ void fold_rows(const uint64_t *scol, const uint32_t *
             const uint32_t sh[2] = {(uint32_t)sr, (uint32_t)(sr >> 32)};
 #pragma unroll
             for (int i = 0; i < SK_IPL; ++i) {
-                const gf16::planar::Multiples m0(ring[d][i].x), m1(ring[d][i].y);
+                // the 16 multiples of each word, then each product is the
+                // multiple its (uniform) scalar selects
+                const SixteenMultiples m0(ring[d][i].x), m1(ring[d][i].y);
 #pragma unroll
                 for (int a = 0; a < K; ++a) {
                     const int bit = 4 * a;
-                    const uint32_t mk0 = gf16::bitmask(sh[bit >> 5], bit & 31);
-                    const uint32_t mk1 = gf16::bitmask(sh[(bit + 1) >> 5], (bit + 1) & 31);
-                    const uint32_t mk2 = gf16::bitmask(sh[(bit + 2) >> 5], (bit + 2) & 31);
-                    const uint32_t mk3 = gf16::bitmask(sh[(bit + 3) >> 5], (bit + 3) & 31);
-                    U[i][a][0] = gf16::mac(U[i][a][0], m0.x0, m0.x1, m0.x2, m0.x3, mk0, mk1, mk2, mk3);
-                    U[i][a][1] = gf16::mac(U[i][a][1], m1.x0, m1.x1, m1.x2, m1.x3, mk0, mk1, mk2, mk3);
+                    const uint32_t sa = bit + 4 <= 32 ? (sh[0] >> bit) & 15u
+                                        : bit >= 32 ? (sh[1] >> (bit - 32)) & 15u
+                                                    : ((sh[0] >> bit) | (sh[1] << (32 - bit))) & 15u;
+                    U[i][a][0] ^= m0.m[sa];
+                    U[i][a][1] ^= m1.m[sa];
                 }
             }
             load(r + FOLD_AHEAD, ring[d]);
@@ -560,7 +584,7 @@ __global__ void __launch_bounds__(64 * FOLD_W) k_sign_fold(Acc acc, size_t n, si
     const int lane = lane_here(), wave = (int)uni(threadIdx.x >> 6);
     const size_t k = base + (size_t)blockIdx.x * FOLD_W + wave;
     if (k >= end) return;  // whole wavefronts; no workgroup barrier follows
-    const size_t item = acc.order(k);
+    const size_t item = uni64(acc.order(k));
     SkFold &A = F[wave];
     const SignItem it = acc(item);
     const uint8_t *vr = it.scratch + SCR_VR;
@@ -569,6 +593,7 @@ __global__ void __launch_bounds__(64 * FOLD_W) k_sign_fold(Acc acc, size_t n, si
         for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(vr + j * V_BYTES, c) << (4 * j);
         A.scol[c] = v;
     }
+    for (int q = lane; q < MW; q += 64) A.t[q] = mvec::load_mw(it.scratch + SCR_T, q);
     WaveSync::sync();
     uint32_t U[SK_IPL][K][SK_MWL];
     fold_rows<V>(A.scol, acc.trow(item), U);
@@ -581,7 +606,7 @@ __global__ void __launch_bounds__(64 * FOLD_W) k_sign_fold(Acc acc, size_t n, si
         reduce_unreduced(a, out);
         uint32_t *yv = (uint32_t *)(it.scratch + SCR_Y);
 #pragma unroll
-        for (int q = 0; q < MW; ++q) yv[q] = out[q] ^ mvec::load_mw(it.scratch + SCR_T, q);
+        for (int q = 0; q < MW; ++q) yv[q] = out[q] ^ A.t[q];
     }
 }
 
@@ -593,13 +618,19 @@ __global__ void __launch_bounds__(64 * FOLD_W) k_verify_fold(Acc acc, size_t n) 
     const int lane = lane_here(), wave = (int)uni(threadIdx.x >> 6);
     const size_t k = (size_t)blockIdx.x * FOLD_W + wave;
     if (k >= n) return;
-    const size_t item = acc.order(k);
+    const size_t item = uni64(acc.order(k));
     SkFold &A = F[wave];
     const VerifyItem it = acc(item);
     for (int c = lane; c < N; c += 64) {
         uint64_t v = 0;
         for (int j = 0; j < K; ++j) v |= (uint64_t)mvec::nib(it.sig, j * N + c) << (4 * j);
         A.scol[c] = v;
+    }
+    for (int w = lane; w < MW; w += 64) {
+        const int nv = mvec::word_valid(w);
+        uint32_t tw = 0;
+        for (int b = 0; b < nv; ++b) tw |= (uint32_t)it.t[4 * w + b] << (8 * b);
+        A.t[w] = tw;
     }
     WaveSync::sync();
     uint32_t U[SK_IPL][K][SK_MWL];
@@ -613,12 +644,7 @@ __global__ void __launch_bounds__(64 * FOLD_W) k_verify_fold(Acc acc, size_t n) 
         reduce_unreduced(a, y);
         bool eq = true;
 #pragma unroll
-        for (int w = 0; w < MW; ++w) {
-            const int nv = mvec::word_valid(w);
-            uint32_t tw = 0;
-            for (int b = 0; b < nv; ++b) tw |= (uint32_t)it.t[4 * w + b] << (8 * b);
-            eq &= (y[w] == tw);
-        }
+        for (int w = 0; w < MW; ++w) eq &= (y[w] == A.t[w]);
         it.result[0] = eq ? 0 : 1;
     }
 }
