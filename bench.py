@@ -46,10 +46,12 @@ PEAK_GF16_TMACS = 256 * 64 * 2.4e9 * 2 / 1e12
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the signing linear-algebra pass, measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (scaled x2.00 and x1.00 by the
-# same counters over a 256 MiB copy) on the default workload; the run is
-# summarised in profiles/sign_verify_mayo1_65536x64keys.md
-STAGE2_TRAFFIC = {(1, 65536, 64): 578_613_440}
-STAGE2_TRAFFIC_AT = "b13bc2a"  # the commit the counter runs were made at
+# same counters over a 256 MiB copy) on the default workload, per launch form
+# (wave: one wavefront per item through the whole quadratic form; shared:
+# the shared-key table and fold kernels); the runs are summarised in
+# profiles/sign_verify_mayo1_65536x64keys.md
+STAGE2_TRAFFIC = {(1, 65536, 64, "wave"): 578_613_440}
+STAGE2_TRAFFIC_AT = {"wave": "b13bc2a"}  # the commit each counter run was made at
 
 
 def make_workload(n: int, nkeys: int, lo: int, hi: int, tag: str = "cfg2"):
@@ -593,6 +595,19 @@ def main():
     # the esk blocks it touches once per signature (L, P1, O) and writes the
     # signature. Keys are shared, so this is an upper bound on compulsory bytes.
     algo_bytes = n * (z.sig_bytes + z.k * z.n // 2 + z.m // 2)
+    lc = eng.launch_config()
+    form = "shared" if lc["sign_prep"] >= 1000 and lc["sign_chunks"] != 0 else "wave"
+    if form == "shared":
+        kernels = (f"signing linear-algebra pass, shared-key form (launch code {lc['sign_prep']}): k_key_sort "
+                   "(items grouped by key), k_sign_table_shared (M and T_j = P1 V_j: products read from LDS "
+                   "tables of the key's entry multiples, built once per workgroup of items of one key; T to "
+                   "the workspace), k_sign_fold (U and y from T, one wavefront per item), k_sign_solve "
+                   "(elimination, one wavefront per item), k_sign_retry (failed attempts); achieved and "
+                   "kernel_ms cover the launches together")
+    else:
+        kernels = ("signing linear-algebra pass: k_sign_prep (M, U, y) + k_sign_solve (elimination), one "
+                   "wavefront per signature, + k_sign_retry (failed attempts, one workgroup per listed "
+                   "item); achieved and kernel_ms cover the three launches together")
     out = {
         "metric": f"MAYO_{v} sign/s & verify/s at batch {n // 1024}K (bit-exact); serve p50/p99 latency",
         "value": round(sign_per_s, 1),
@@ -620,10 +635,8 @@ def main():
                          "max_attempts": int(attempts.max())},
         "roofline": {
             "bound": "valu",
-            "kernel": "signing linear-algebra pass: k_sign_prep (M, U, y) + k_sign_solve "
-                      "(elimination), one wavefront per signature, + k_sign_retry (failed "
-                      "attempts, one workgroup per listed item); achieved and kernel_ms cover "
-                      "the three launches together",
+            "kernel": kernels,
+            "launch_form": form,
             "achieved": round(achieved_tmacs, 3),
             "peak": round(PEAK_GF16_TMACS, 1),
             "unit": "T GF16-MAC/s",
@@ -632,9 +645,9 @@ def main():
             "macs_per_sign": SIGN_MACS[v],
             "hbm_algorithmic_GBs": round(algo_bytes / (main_ms * 1e-3) / 1e9, 2),
             "hbm_peak_GBs": HBM_PEAK_GBS,
-            "traffic": STAGE2_TRAFFIC.get((v, n, args.keys)),
+            "traffic": STAGE2_TRAFFIC.get((v, n, args.keys, form)),
             "traffic_unit": "bytes per launch (PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "traffic_measured_at": STAGE2_TRAFFIC_AT if (v, n, args.keys) in STAGE2_TRAFFIC else None,
+            "traffic_measured_at": STAGE2_TRAFFIC_AT.get(form) if (v, n, args.keys, form) in STAGE2_TRAFFIC else None,
         },
     }
 
