@@ -501,6 +501,76 @@ struct SixteenMultiples {
     }
 };
 
+// One row of the fold for one fold item: U[a][h] += S_a[r] x_h for the K
+// scalars S_a[r] (nibble a of the uniform lo:hi) and the item's two words
+// x_0, x_1. The 16 multiples s x_h are built in v0..v31 (multiple s of word
+// h in v[2 s + h]; x_h itself arrives in v2 / v3), and each product is one
+// XOR whose first operand is read relative to the scalar (GPR indexing mode,
+// index 2 S_a[r]): 63 VALU and 32 SALU instructions per row against 4
+// AND-XORs per product. Registers only; no memory access. mul_a (s -> a s in the planar
+// layout, a^4 = a + 1): the bytes (p0, p1, p2, p3) -> (p3, p0 ^ p3, p1, p2),
+// a rotation and the top byte XORed into byte 1.
+#define SKF_MULA(SRC, DST)                                                                      \
+    "v_alignbit_b32 " DST ", " SRC ", " SRC ", 24\n\t"                                        \
+    "v_lshlrev_b32_sdwa v1, v0, " SRC " dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "  \
+    "src1_sel:BYTE_3\n\t"                                                                      \
+    "v_xor_b32 " DST ", " DST ", v1\n\t"
+#define SKF_MULTIPLES                                                                           \
+    "v_mov_b32 v0, 8\n\t"                                                                       \
+    SKF_MULA("v2", "v4") SKF_MULA("v4", "v8") SKF_MULA("v8", "v16")                             \
+    SKF_MULA("v3", "v5") SKF_MULA("v5", "v9") SKF_MULA("v9", "v17")                             \
+    "v_xor_b32 v6, v2, v4\n\tv_xor_b32 v7, v3, v5\n\t"                                          \
+    "v_xor_b32 v10, v2, v8\n\tv_xor_b32 v11, v3, v9\n\t"                                        \
+    "v_xor_b32 v12, v4, v8\n\tv_xor_b32 v13, v5, v9\n\t"                                        \
+    "v_xor_b32 v14, v6, v8\n\tv_xor_b32 v15, v7, v9\n\t"                                        \
+    "v_xor_b32 v18, v2, v16\n\tv_xor_b32 v19, v3, v17\n\t"                                      \
+    "v_xor_b32 v20, v4, v16\n\tv_xor_b32 v21, v5, v17\n\t"                                      \
+    "v_xor_b32 v22, v6, v16\n\tv_xor_b32 v23, v7, v17\n\t"                                      \
+    "v_xor_b32 v24, v8, v16\n\tv_xor_b32 v25, v9, v17\n\t"                                      \
+    "v_xor_b32 v26, v10, v16\n\tv_xor_b32 v27, v11, v17\n\t"                                    \
+    "v_xor_b32 v28, v12, v16\n\tv_xor_b32 v29, v13, v17\n\t"                                    \
+    "v_xor_b32 v30, v14, v16\n\tv_xor_b32 v31, v15, v17\n\t"                                    \
+    "v_mov_b32 v0, 0\n\tv_mov_b32 v1, 0\n\t"
+// product a: index 2 (nibble a of lo:hi), then U[a] ^= the two indexed
+// words. Indexing mode is switched on before product 0 (with an index of 0)
+// and off after the last; in between only these XORs run on the VALU.
+#define SKF_A(A, SRC, BFE)                                                                      \
+    "s_bfe_u32 %[t], %[" #SRC "], " #BFE "\n\t"                                                 \
+    "s_lshl_b32 %[t], %[t], 1\n\t"                                                              \
+    "s_set_gpr_idx_idx %[t]\n\t"                                                                \
+    "v_xor_b32 %[u" #A "a], v0, %[u" #A "a]\n\t"                                                \
+    "v_xor_b32 %[u" #A "b], v1, %[u" #A "b]\n\t"
+#define SKF_OP(A) [u##A##a] "+v"(U[A][0]), [u##A##b] "+v"(U[A][1])
+#define SKF_A0_3 SKF_A(0, lo, 0x40000) SKF_A(1, lo, 0x40004) SKF_A(2, lo, 0x40008) SKF_A(3, lo, 0x4000c)
+#define SKF_A4_7 SKF_A(4, lo, 0x40010) SKF_A(5, lo, 0x40014) SKF_A(6, lo, 0x40018) SKF_A(7, lo, 0x4001c)
+#if MAYO_VERSION == 2
+#define SKF_PRODUCTS SKF_A0_3
+#define SKF_OPS SKF_OP(0), SKF_OP(1), SKF_OP(2), SKF_OP(3)
+#elif MAYO_VERSION == 1
+#define SKF_PRODUCTS SKF_A0_3 SKF_A4_7 SKF_A(8, hi, 0x40000) SKF_A(9, hi, 0x40004)
+#define SKF_OPS SKF_OP(0), SKF_OP(1), SKF_OP(2), SKF_OP(3), SKF_OP(4), SKF_OP(5), SKF_OP(6), SKF_OP(7), \
+                SKF_OP(8), SKF_OP(9)
+#elif MAYO_VERSION == 3
+#define SKF_PRODUCTS SKF_A0_3 SKF_A4_7 SKF_A(8, hi, 0x40000) SKF_A(9, hi, 0x40004) SKF_A(10, hi, 0x40008)
+#define SKF_OPS SKF_OP(0), SKF_OP(1), SKF_OP(2), SKF_OP(3), SKF_OP(4), SKF_OP(5), SKF_OP(6), SKF_OP(7), \
+                SKF_OP(8), SKF_OP(9), SKF_OP(10)
+#else
+#define SKF_PRODUCTS \
+    SKF_A0_3 SKF_A4_7 SKF_A(8, hi, 0x40000) SKF_A(9, hi, 0x40004) SKF_A(10, hi, 0x40008) SKF_A(11, hi, 0x4000c)
+#define SKF_OPS SKF_OP(0), SKF_OP(1), SKF_OP(2), SKF_OP(3), SKF_OP(4), SKF_OP(5), SKF_OP(6), SKF_OP(7), \
+                SKF_OP(8), SKF_OP(9), SKF_OP(10), SKF_OP(11)
+#endif
+__device__ __forceinline__ void fold_row_asm(uint32_t (&U)[K][SK_MWL], uint2 x, uint32_t lo, uint32_t hi) {
+    uint32_t x0 = x.x, x1 = x.y, t;
+    asm volatile(SKF_MULTIPLES "s_mov_b32 %[t], 0\n\ts_set_gpr_idx_on %[t], gpr_idx(SRC0)\n\t" SKF_PRODUCTS
+                 "s_set_gpr_idx_off"
+                 : [t] "=&s"(t), "+{v2}"(x0), "+{v3}"(x1), SKF_OPS
+                 : [lo] "s"(lo), [hi] "s"(hi)
+                 : "v0", "v1", "v4", "v5", "v6", "v7", "v8", "v9", "v10", "v11", "v12", "v13", "v14", "v15",
+                   "v16", "v17", "v18", "v19", "v20", "v21", "v22", "v23", "v24", "v25", "v26", "v27", "v28",
+                   "v29", "v30", "v31", "m0", "scc");
+}
+
 template <int NP>
 __device__ __forceinline__ void fold_rows(const uint64_t *scol, const uint32_t *Tg, uint32_t (&U)[SK_IPL][K][SK_MWL]) {
     const int lane = lane_here();
@@ -534,20 +604,7 @@ __device__ __forceinline__ void fold_rows(const uint64_t *scol, const uint32_t *
             const uint64_t sr = uni64(r < NP ? scol[r] : 0ull);
             const uint32_t sh[2] = {(uint32_t)sr, (uint32_t)(sr >> 32)};
 #pragma unroll
-            for (int i = 0; i < SK_IPL; ++i) {
-                // the 16 multiples of each word, then each product is the
-                // multiple its (uniform) scalar selects
-                const SixteenMultiples m0(ring[d][i].x), m1(ring[d][i].y);
-#pragma unroll
-                for (int a = 0; a < K; ++a) {
-                    const int bit = 4 * a;
-                    const uint32_t sa = bit + 4 <= 32 ? (sh[0] >> bit) & 15u
-                                        : bit >= 32 ? (sh[1] >> (bit - 32)) & 15u
-                                                    : ((sh[0] >> bit) | (sh[1] << (32 - bit))) & 15u;
-                    U[i][a][0] ^= m0.m[sa];
-                    U[i][a][1] ^= m1.m[sa];
-                }
-            }
+            for (int i = 0; i < SK_IPL; ++i) fold_row_asm(U[i], ring[d][i], sh[0], sh[1]);
             load(r + FOLD_AHEAD, ring[d]);
         }
     }
