@@ -532,8 +532,8 @@ class Engine:
     def launch_variants(self) -> dict:
         out = {}
         for name, f in LAUNCH_FIELDS.items():
-            buf = (ctypes.c_int32 * 8)()
-            cnt = self.lib.cumayo_launch_variants(f, buf, 8)
+            buf = (ctypes.c_int32 * 32)()
+            cnt = min(self.lib.cumayo_launch_variants(f, buf, 32), 32)
             out[name] = [int(buf[i]) for i in range(cnt)]
         return out
 

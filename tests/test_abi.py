@@ -75,9 +75,9 @@ def test_launch_variants_and_config(version):
     lib = load_library(version)
     variants = {}
     for name, f in LAUNCH_FIELDS.items():
-        buf = (ctypes.c_int32 * 8)()
-        cnt = lib.cumayo_launch_variants(f, buf, 8)
-        assert 2 <= cnt <= 8, name
+        buf = (ctypes.c_int32 * 16)()
+        cnt = lib.cumayo_launch_variants(f, buf, 16)
+        assert 2 <= cnt <= 16, name
         variants[name] = [buf[i] for i in range(cnt)]
         assert len(set(variants[name])) == cnt, (name, variants[name])
     assert lib.cumayo_launch_variants(99, None, 0) == -1
