@@ -506,7 +506,7 @@ struct SixteenMultiples {
 // x_0, x_1. The 16 multiples s x_h are built in v0..v31 (multiple s of word
 // h in v[2 s + h]; x_h itself arrives in v2 / v3), and each product is one
 // XOR whose first operand is read relative to the scalar (GPR indexing mode,
-// index 2 S_a[r]): 63 VALU and 32 SALU instructions per row against 4
+// index 2 S_a[r]): 63 VALU and 40 SALU instructions per row against 4
 // AND-XORs per product. Registers only; no memory access. mul_a (s -> a s in the planar
 // layout, a^4 = a + 1): the bytes (p0, p1, p2, p3) -> (p3, p0 ^ p3, p1, p2),
 // a rotation and the top byte XORed into byte 1.
@@ -532,14 +532,16 @@ struct SixteenMultiples {
     "v_xor_b32 v30, v14, v16\n\tv_xor_b32 v31, v15, v17\n\t"                                    \
     "v_mov_b32 v0, 0\n\tv_mov_b32 v1, 0\n\t"
 // product a: index 2 (nibble a of lo:hi), then U[a] ^= the two indexed
-// words. Indexing mode is switched on before product 0 (with an index of 0)
-// and off after the last; in between only these XORs run on the VALU.
+// words, indexing mode switched on and off around them (the form the
+// compiler itself emits for a register-indexed read; the index is not
+// switched while the mode is on)
 #define SKF_A(A, SRC, BFE)                                                                      \
     "s_bfe_u32 %[t], %[" #SRC "], " #BFE "\n\t"                                                 \
     "s_lshl_b32 %[t], %[t], 1\n\t"                                                              \
-    "s_set_gpr_idx_idx %[t]\n\t"                                                                \
+    "s_set_gpr_idx_on %[t], gpr_idx(SRC0)\n\t"                                                  \
     "v_xor_b32 %[u" #A "a], v0, %[u" #A "a]\n\t"                                                \
-    "v_xor_b32 %[u" #A "b], v1, %[u" #A "b]\n\t"
+    "v_xor_b32 %[u" #A "b], v1, %[u" #A "b]\n\t"                                                \
+    "s_set_gpr_idx_off\n\t"
 #define SKF_OP(A) [u##A##a] "+v"(U[A][0]), [u##A##b] "+v"(U[A][1])
 #define SKF_A0_3 SKF_A(0, lo, 0x40000) SKF_A(1, lo, 0x40004) SKF_A(2, lo, 0x40008) SKF_A(3, lo, 0x4000c)
 #define SKF_A4_7 SKF_A(4, lo, 0x40010) SKF_A(5, lo, 0x40014) SKF_A(6, lo, 0x40018) SKF_A(7, lo, 0x4001c)
@@ -562,8 +564,7 @@ struct SixteenMultiples {
 #endif
 __device__ __forceinline__ void fold_row_asm(uint32_t (&U)[K][SK_MWL], uint2 x, uint32_t lo, uint32_t hi) {
     uint32_t x0 = x.x, x1 = x.y, t;
-    asm volatile(SKF_MULTIPLES "s_mov_b32 %[t], 0\n\ts_set_gpr_idx_on %[t], gpr_idx(SRC0)\n\t" SKF_PRODUCTS
-                 "s_set_gpr_idx_off"
+    asm volatile(SKF_MULTIPLES SKF_PRODUCTS
                  : [t] "=&s"(t), "+{v2}"(x0), "+{v3}"(x1), SKF_OPS
                  : [lo] "s"(lo), [hi] "s"(hi)
                  : "v0", "v1", "v4", "v5", "v6", "v7", "v8", "v9", "v10", "v11", "v12", "v13", "v14", "v15",
