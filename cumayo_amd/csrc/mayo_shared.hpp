@@ -45,6 +45,7 @@ constexpr int SORT_BINS = 4096;
 template <class Acc>
 __global__ void __launch_bounds__(1024) k_key_sort(Acc acc, size_t n) {
     constexpr int PER = SORT_BINS / 1024;
+    constexpr int KU = 8;  // keys per thread loaded together in each pass
     __shared__ uint32_t pos[SORT_BINS];
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t kmax;
@@ -52,13 +53,32 @@ __global__ void __launch_bounds__(1024) k_key_sort(Acc acc, size_t n) {
     if (t == 0) kmax = 0;
     for (int b = t; b < SORT_BINS; b += 1024) pos[b] = 0;
     __syncthreads();
+    // keys b0 + t + 1024 u, u < KU, all loads issued before any is used
+    auto keys = [&](size_t b0, uint32_t (&kv)[KU]) {
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const size_t i = b0 + t + 1024 * (size_t)u;
+            kv[u] = i < n ? acc.key(i) : 0u;
+        }
+    };
     uint32_t km = 0;
-    for (size_t i = t; i < n; i += 1024) km = max(km, acc.key(i));
+    for (size_t b0 = 0; b0 < n; b0 += 1024 * KU) {
+        uint32_t kv[KU];
+        keys(b0, kv);
+#pragma unroll
+        for (int u = 0; u < KU; ++u) km = max(km, kv[u]);
+    }
     atomicMax(&kmax, km);
     __syncthreads();
     const int bits = kmax ? 32 - __builtin_clz(kmax) : 0;
     const int sh = bits > 12 ? bits - 12 : 0;  // key >> sh < SORT_BINS
-    for (size_t i = t; i < n; i += 1024) atomicAdd(&pos[acc.key(i) >> sh], 1u);
+    for (size_t b0 = 0; b0 < n; b0 += 1024 * KU) {
+        uint32_t kv[KU];
+        keys(b0, kv);
+#pragma unroll
+        for (int u = 0; u < KU; ++u)
+            if (b0 + t + 1024 * (size_t)u < n) atomicAdd(&pos[kv[u] >> sh], 1u);
+    }
     __syncthreads();
     // exclusive scan: thread t owns bins PER t .. PER t + PER - 1
     uint32_t v[PER], s = 0;
@@ -83,9 +103,14 @@ __global__ void __launch_bounds__(1024) k_key_sort(Acc acc, size_t n) {
         before += v[u];
     }
     __syncthreads();
-    for (size_t i = t; i < n; i += 1024) {
-        const uint32_t k = atomicAdd(&pos[acc.key(i) >> sh], 1u);
-        acc.set_order(k, (uint32_t)i);
+    for (size_t b0 = 0; b0 < n; b0 += 1024 * KU) {
+        uint32_t kv[KU];
+        keys(b0, kv);
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const size_t i = b0 + t + 1024 * (size_t)u;
+            if (i < n) acc.set_order(atomicAdd(&pos[kv[u] >> sh], 1u), (uint32_t)i);
+        }
     }
 }
 

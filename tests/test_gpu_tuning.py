@@ -106,6 +106,47 @@ def test_every_launch_variant_gives_the_same_bytes(engine_factory, restore_confi
         assert r[i].tobytes() == O.sign(v, esk[b["kidx"][i]].tobytes(), m)[0], i
 
 
+@pytest.mark.parametrize("nkeys", (40, 5000), ids=("40keys", "5000keys"))
+def test_shared_key_kernels_over_several_sort_batches(engine_factory, restore_config, nkeys):
+    """The shared-key kernels on 20,000 items (the key sort loads its keys in
+    batches of 8,192, so three of them, the last ragged) give the wave
+    kernels' bytes: with 40 keys (about 500 items per key) and with 5,000
+    keys, past the sort's 4,096 exact bins, where a workgroup holds items of
+    several keys and runs its table loop once per key."""
+    import torch
+    v = 1
+    eng = restore_config(engine_factory(v))
+    z = eng.sizes
+    n = 20000
+    b = _batch(eng, n, nkeys, 90 + nkeys % 7, maxlen=200)
+    var = eng.launch_variants()
+    wave_p = [p for p in var["sign_prep"] if p < 1000][0]
+    wave_v = [p for p in var["verify"] if p < 1000][0]
+    shared = [p for p in var["sign_prep"] if p >= 1000]
+    print(f"[tuning] shared-key kernels, {n} items, {nkeys} keys, codes {shared}", flush=True)
+    ws, wv = eng.sign_workspace(n), eng.verify_workspace(n)
+    eng.set_launch_config(sign_prep=wave_p, sign_order=0, sign_chunks=1, verify=wave_v, verify_order=0)
+    ref = torch.zeros((n, z.sig_bytes), dtype=torch.uint8, device=eng.device)
+    eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], ref, ws)
+    bad = ref.clone()
+    bad[::3, 5] ^= 0x21
+    vref = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+    eng.verify_prepared(b["pepk"], b["k"], b["m"], b["o"], b["l"], bad, vref, wv)
+    torch.cuda.synchronize()
+    expect = np.zeros(n, np.uint8)
+    expect[::3] = 1
+    assert np.array_equal(vref.cpu().numpy(), expect)
+    for p, chunks in itertools.product(shared, (1, 2)):
+        eng.set_launch_config(sign_prep=p, sign_chunks=chunks, verify=p)
+        out = torch.full_like(ref, 0xA5)
+        eng.sign_prepared(b["pesk"], b["k"], b["m"], b["o"], b["l"], out, ws)
+        res = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
+        eng.verify_prepared(b["pepk"], b["k"], b["m"], b["o"], b["l"], bad, res, wv)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), (p, chunks)
+        assert torch.equal(res, vref), (p, chunks)
+
+
 def test_autotune_keeps_a_compiled_variant_and_the_results(engine_factory, restore_config):
     import torch
     v = 1
