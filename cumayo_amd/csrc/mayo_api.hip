@@ -418,7 +418,9 @@ void cumayo_test_pitched_fail_first(int attempts) {
     g_pitched_fail_first.store(attempts > 0 ? (attempts < MAX_ATTEMPTS ? attempts : MAX_ATTEMPTS) : 0);
 }
 
-size_t cumayo_sign_workspace_size(size_t n) { return n * ((size_t)SIGN_SCRATCH_STRIDE + SIGN_T_BYTES); }
+size_t cumayo_sign_workspace_size(size_t n) {
+    return n * ((size_t)SIGN_SCRATCH_STRIDE + SIGN_T_BYTES) + wv::key_sort_bytes(n);
+}
 
 }  // extern "C"
 
@@ -575,14 +577,15 @@ int sign_table_impl(const TableSign &acc, size_t n, hipStream_t stream, int stag
             k_sign_main<TableSign><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
             const int fail_first = (stages >> 8) & 0xFF;
-            // shared-key tables (prepared keys, split launches, a T region in
-            // the workspace): items in key order, several per workgroup
+            // shared-key tables (prepared keys, split launches, a 16-byte
+            // aligned T region in the workspace): items in key order, several
+            // per workgroup
             const bool shared = std::is_same<KL, wv::PrepKeys>::value && cfg.sign_chunks != 0 &&
-                                wv::sk_code(cfg.prep) && acc.trows;
+                                wv::sk_code(cfg.prep) && acc.trows && !((uintptr_t)acc.trows & 15);
             const int ordered = shared || (cfg.sign_order && n > 64);
             const int parts = cfg.sign_chunks > 1 && n >= CHUNK_MIN ? 2 : 1;
             if (shared)
-                wv::k_key_sort<TableSign><<<1, 1024, 0, stream>>>(acc, n);
+                wv::launch_key_sort(acc, n, (uint32_t *)(acc.trows + n * SIGN_T_BYTES), stream);
             else if (ordered)
                 k_key_order<TableSign><<<1, 1024, 0, stream>>>(acc, n);
             if (cfg.sign_chunks == 0) {
@@ -660,10 +663,11 @@ int verify_table_impl(const TableVerify &acc, size_t n, hipStream_t stream, int 
         if (stages & 4) {
             k_verify_main<TableVerify><<<(unsigned)n, WG, 0, stream>>>(acc, n);
         } else {
-            const bool shared = std::is_same<KL, wv::PrepKeys>::value && wv::sk_code(cfg.verify) && acc.trows;
+            const bool shared = std::is_same<KL, wv::PrepKeys>::value && wv::sk_code(cfg.verify) && acc.trows &&
+                                !((uintptr_t)acc.trows & 15);
             const int ordered = !shared && cfg.verify_order && n > 64;
             if (shared)
-                wv::k_key_sort<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
+                wv::launch_key_sort(acc, n, (uint32_t *)(acc.trows + n * VERIFY_T_BYTES), stream);
             else if (ordered)
                 k_key_order<TableVerify><<<1, 1024, 0, stream>>>(acc, n);
             if constexpr (std::is_same<KL, wv::PrepKeys>::value) {
@@ -842,7 +846,9 @@ int cumayo_sign_table(const uint8_t *esk_table, size_t esk_stride, const int32_t
                                     workspace, stream, 3);
 }
 
-size_t cumayo_verify_workspace_size(size_t n) { return n * ((size_t)VERIFY_SCRATCH_STRIDE + VERIFY_T_BYTES); }
+size_t cumayo_verify_workspace_size(size_t n) {
+    return n * ((size_t)VERIFY_SCRATCH_STRIDE + VERIFY_T_BYTES) + wv::key_sort_bytes(n);
+}
 
 int cumayo_verify_table_stages(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
                                const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,

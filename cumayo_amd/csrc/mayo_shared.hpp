@@ -28,7 +28,7 @@
 // (sign.cu:395-437, verify.cu:72-182).
 //
 // The items of a workgroup are W consecutive positions of the key order
-// (k_key_sort), so they share a key except at the boundaries between keys;
+// (launch_key_sort), so they share a key except at the boundaries between keys;
 // a workgroup whose items hold several keys runs the table loop once per
 // key, each wavefront reading in the pass for its own key.
 #pragma once
@@ -38,55 +38,66 @@ namespace mayo {
 namespace wv {
 
 // ---- key order ------------------------------------------------------------------
-// Items sorted by key (counting sort over up to SORT_BINS bins: exact for key
-// ranges up to 4,096, the top 12 bits of the key above), sorted position k
-// to acc.set_order(k). One workgroup; positions within a key in any order.
+// Items sorted by key: a counting sort over SORT_BINS bins (key mod 4,096:
+// exact for key ranges up to 4,096), sorted position k to acc.set_order(k),
+// positions within a bin in any order. Three kernels over workgroups of
+// KS_ITEMS items: per-workgroup histograms, one workgroup that turns them
+// into each workgroup's first position per bin, and the scatter. `hist`
+// holds SORT_BINS words per workgroup (key_sort_bytes(n), the end of the
+// table-form workspaces, after the T regions).
 constexpr int SORT_BINS = 4096;
+constexpr int KS_ITEMS = 4096;  // items per workgroup, 4 per thread
+constexpr int KS_U = KS_ITEMS / 1024;
+inline unsigned key_sort_groups(size_t n) { return (unsigned)((n + KS_ITEMS - 1) / KS_ITEMS); }
+inline size_t key_sort_bytes(size_t n) { return (size_t)key_sort_groups(n) * SORT_BINS * 4; }
+
 template <class Acc>
-__global__ void __launch_bounds__(1024) k_key_sort(Acc acc, size_t n) {
-    constexpr int PER = SORT_BINS / 1024;
-    constexpr int KU = 8;  // keys per thread loaded together in each pass
-    __shared__ uint32_t pos[SORT_BINS];
+__device__ __forceinline__ void ks_keys(const Acc &acc, size_t n, uint32_t (&kv)[KS_U]) {
+#pragma unroll
+    for (int u = 0; u < KS_U; ++u) {
+        const size_t i = (size_t)blockIdx.x * KS_ITEMS + threadIdx.x + 1024 * u;
+        kv[u] = i < n ? acc.key(i) % SORT_BINS : 0u;
+    }
+}
+
+template <class Acc>
+__global__ void __launch_bounds__(1024) k_key_hist(Acc acc, size_t n, uint32_t *hist) {
+    __shared__ uint32_t h[SORT_BINS];
+    const int t = threadIdx.x;
+    for (int b = t; b < SORT_BINS; b += 1024) h[b] = 0;
+    uint32_t kv[KS_U];
+    ks_keys(acc, n, kv);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KS_U; ++u)
+        if ((size_t)blockIdx.x * KS_ITEMS + t + 1024 * u < n) atomicAdd(&h[kv[u]], 1u);
+    __syncthreads();
+    for (int b = t; b < SORT_BINS; b += 1024) hist[(size_t)blockIdx.x * SORT_BINS + b] = h[b];
+}
+
+// hist[g][b] (counts) -> the first sorted position of workgroup g's items
+// of bin b: bins in order, within a bin the workgroups in order
+__global__ void __launch_bounds__(1024) k_key_scan(uint32_t *hist, unsigned groups) {
+    constexpr int PER = SORT_BINS / 1024;  // thread t: bins PER t .. PER t + PER - 1
+    constexpr int GB = 8;                  // workgroups' counts loaded together
+    static_assert(PER == 4, "one 16-byte load per workgroup");
     __shared__ uint32_t wsum[16];
-    __shared__ uint32_t kmax;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (t == 0) kmax = 0;
-    for (int b = t; b < SORT_BINS; b += 1024) pos[b] = 0;
-    __syncthreads();
-    // keys b0 + t + 1024 u, u < KU, all loads issued before any is used
-    auto keys = [&](size_t b0, uint32_t (&kv)[KU]) {
+    auto row = [&](unsigned g) { return (uint4 *)(hist + (size_t)g * SORT_BINS) + t; };
+    uint32_t tot[PER] = {0u, 0u, 0u, 0u};
+    for (unsigned g0 = 0; g0 < groups; g0 += GB) {
+        uint4 c[GB];
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
-            const size_t i = b0 + t + 1024 * (size_t)u;
-            kv[u] = i < n ? acc.key(i) : 0u;
+        for (int gg = 0; gg < GB; ++gg) c[gg] = g0 + gg < groups ? *row(g0 + gg) : uint4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int gg = 0; gg < GB; ++gg) {
+            tot[0] += c[gg].x;
+            tot[1] += c[gg].y;
+            tot[2] += c[gg].z;
+            tot[3] += c[gg].w;
         }
-    };
-    uint32_t km = 0;
-    for (size_t b0 = 0; b0 < n; b0 += 1024 * KU) {
-        uint32_t kv[KU];
-        keys(b0, kv);
-#pragma unroll
-        for (int u = 0; u < KU; ++u) km = max(km, kv[u]);
     }
-    atomicMax(&kmax, km);
-    __syncthreads();
-    const int bits = kmax ? 32 - __builtin_clz(kmax) : 0;
-    const int sh = bits > 12 ? bits - 12 : 0;  // key >> sh < SORT_BINS
-    for (size_t b0 = 0; b0 < n; b0 += 1024 * KU) {
-        uint32_t kv[KU];
-        keys(b0, kv);
-#pragma unroll
-        for (int u = 0; u < KU; ++u)
-            if (b0 + t + 1024 * (size_t)u < n) atomicAdd(&pos[kv[u] >> sh], 1u);
-    }
-    __syncthreads();
-    // exclusive scan: thread t owns bins PER t .. PER t + PER - 1
-    uint32_t v[PER], s = 0;
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        v[u] = pos[PER * t + u];
-        s += v[u];
-    }
+    const uint32_t s = tot[0] + tot[1] + tot[2] + tot[3];
     uint32_t incl = s;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -95,23 +106,50 @@ __global__ void __launch_bounds__(1024) k_key_sort(Acc acc, size_t n) {
     }
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    uint32_t before = incl - s;
-    for (int k = 0; k < w; ++k) before += wsum[k];
+    uint32_t run[PER];
+    run[0] = incl - s;
+    for (int k = 0; k < w; ++k) run[0] += wsum[k];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        pos[PER * t + u] = before;
-        before += v[u];
-    }
-    __syncthreads();
-    for (size_t b0 = 0; b0 < n; b0 += 1024 * KU) {
-        uint32_t kv[KU];
-        keys(b0, kv);
+    for (int u = 1; u < PER; ++u) run[u] = run[u - 1] + tot[u - 1];
+    for (unsigned g0 = 0; g0 < groups; g0 += GB) {
+        uint4 c[GB];
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
-            const size_t i = b0 + t + 1024 * (size_t)u;
-            if (i < n) acc.set_order(atomicAdd(&pos[kv[u] >> sh], 1u), (uint32_t)i);
+        for (int gg = 0; gg < GB; ++gg) c[gg] = g0 + gg < groups ? *row(g0 + gg) : uint4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int gg = 0; gg < GB; ++gg) {
+            if (g0 + gg >= groups) break;
+            *row(g0 + gg) = uint4{run[0], run[1], run[2], run[3]};
+            run[0] += c[gg].x;
+            run[1] += c[gg].y;
+            run[2] += c[gg].z;
+            run[3] += c[gg].w;
         }
     }
+}
+
+template <class Acc>
+__global__ void __launch_bounds__(1024) k_key_scatter(Acc acc, size_t n, const uint32_t *hist) {
+    __shared__ uint32_t pos[SORT_BINS];
+    const int t = threadIdx.x;
+    for (int b = t; b < SORT_BINS; b += 1024) pos[b] = hist[(size_t)blockIdx.x * SORT_BINS + b];
+    uint32_t kv[KS_U];
+    ks_keys(acc, n, kv);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KS_U; ++u) {
+        const size_t i = (size_t)blockIdx.x * KS_ITEMS + t + 1024 * u;
+        if (i < n) acc.set_order(atomicAdd(&pos[kv[u]], 1u), (uint32_t)i);
+    }
+}
+
+// the three kernels on `stream`, with `hist` of key_sort_groups(n) x
+// SORT_BINS words
+template <class Acc>
+inline void launch_key_sort(const Acc &acc, size_t n, uint32_t *hist, hipStream_t stream) {
+    const unsigned g = key_sort_groups(n);
+    hipLaunchKernelGGL(k_key_hist<Acc>, dim3(g), dim3(1024), 0, stream, acc, n, hist);
+    hipLaunchKernelGGL(k_key_scan, dim3(1), dim3(1024), 0, stream, hist, g);
+    hipLaunchKernelGGL(k_key_scatter<Acc>, dim3(g), dim3(1024), 0, stream, acc, n, (const uint32_t *)hist);
 }
 
 // ---- shared-key tables ----------------------------------------------------------

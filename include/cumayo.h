@@ -141,7 +141,8 @@ void cumayo_test_pitched_fail_first(int attempts);
  * pointers are device pointers; workspace holds
  * cumayo_sign_workspace_size(n) bytes: n per-item slots, then one T region
  * per item that the shared-key launch variants write and read (the rows
- * T_j = P1 V_j, 31.2 KB per item for MAYO_1). Returns 0 or a hipError_t. The
+ * T_j = P1 V_j, 31.2 KB per item for MAYO_1), then 16 KB per 4,096 items for
+ * their key sort. Returns 0 or a hipError_t. The
  * per-item arrays are device data and are not checked: key_idx[i] must name
  * a row of the table, and msg_off[i] >= 0, msg_len[i] >= 0 must lie within
  * msgs (as for every table and prepared form below); other values make the
@@ -170,7 +171,8 @@ int cumayo_sign_table_stages(const uint8_t *esk_table, size_t esk_stride, const 
 /* Batched verification over a table of expanded public keys (P1||P2||P3);
  * results[i] = 0 valid, 1 invalid. The workspace holds
  * cumayo_verify_workspace_size(n) bytes: n per-item slots, then one T region
- * per item for the shared-key variants (34.4 KB per item for MAYO_1). */
+ * per item for the shared-key variants (34.4 KB per item for MAYO_1) and
+ * their key sort's 16 KB per 4,096 items. */
 size_t cumayo_verify_workspace_size(size_t n);
 int cumayo_verify_table(const uint8_t *epk_table, size_t epk_stride, const int32_t *key_idx,
                         const uint8_t *msgs, const int64_t *msg_off, const int32_t *msg_len,
