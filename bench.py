@@ -598,10 +598,10 @@ def main():
     lc = eng.launch_config()
     form = "shared" if lc["sign_prep"] >= 1000 and lc["sign_chunks"] != 0 else "wave"
     if form == "shared":
-        kernels = (f"signing linear-algebra pass, shared-key form (launch code {lc['sign_prep']}): k_key_sort "
-                   "(items grouped by key), k_sign_table_shared (M and T_j = P1 V_j: products read from LDS "
-                   "tables of the key's entry multiples, built once per workgroup of items of one key; T to "
-                   "the workspace), k_sign_fold (U and y from T, one wavefront per item), k_sign_solve "
+        kernels = (f"signing linear-algebra pass, shared-key form (launch code {lc['sign_prep']}): the key "
+                   "sort (k_key_hist, k_key_scan, k_key_scatter: items grouped by key), k_sign_table_shared "
+                   "(M and T_j = P1 V_j: products read from LDS tables of the key's entry multiples, built "
+                   "once per workgroup of items of one key; T to the workspace), k_sign_fold (U and y from T, one wavefront per item), k_sign_solve "
                    "(elimination, one wavefront per item), k_sign_retry (failed attempts); achieved and "
                    "kernel_ms cover the launches together")
     else:
