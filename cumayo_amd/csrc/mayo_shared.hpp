@@ -1,5 +1,6 @@
-// Shared-key quadratic forms: a workgroup of W wavefronts signs (or
-// verifies) up to W items under one key, one wavefront per item.
+// Shared-key quadratic forms: a workgroup of W wavefronts computes the
+// quadratic form of up to W items under one key, one wavefront per item, in
+// two kernels: the table kernel (the column loop, below) and the fold.
 //
 // The quadratic form's column loop needs, for every entry P[r][c] of the key
 // and every scalar S_j[c] of an item, the product S_j[c] P[r][c]. A product
@@ -19,18 +20,19 @@
 // [c0, c0 + CG) the producing threads write those of the next CG columns,
 // and one workgroup barrier per CG columns separates the two.
 //
-// At a chunk's end the lanes store T (and, for the first NLR rows of
-// signing, which are the columns x of L, M_j[x] = T_j[x]) and the wave folds
-// the chunk's P rows into the pair sums, U_(a,j) += S_a[r] T_j[r], in a
-// transposed layout: lanes own (j, word pair) items and keep U_(a,j) for all
-// a in registers (2K per item), rows are uniform, so the masks of S_a[r] are
-// SGPR masks. U_(a,j) + U_(j,a) is the pair sum of the bitsliced kernels
+// At a chunk's end the lanes store their rows: for signing, the first NLR
+// rows are the columns x of L, M_j[x] = T_j[x], stored to M; the rows of P
+// go to the item's T region in the workspace. The fold kernel (one
+// wavefront per item) then forms the pair sums, U_(a,j) += S_a[r] T_j[r],
+// in a transposed layout: lanes own (j, word pair) items and keep U_(a,j)
+// for all a in registers (2K per item), and rows are uniform across the
+// wave. U_(a,j) + U_(j,a) is the pair sum of the bitsliced kernels
 // (sign.cu:395-437, verify.cu:72-182).
 //
 // The items of a workgroup are W consecutive positions of the key order
-// (launch_key_sort), so they share a key except at the boundaries between keys;
-// a workgroup whose items hold several keys runs the table loop once per
-// key, each wavefront reading in the pass for its own key.
+// (launch_key_sort), so they share a key except at the boundaries between
+// keys; a workgroup whose items hold several keys runs the table loop once
+// per key, each wavefront reading in the pass for its own key.
 #pragma once
 #include "mayo_wave.hpp"
 
@@ -528,8 +530,9 @@ __global__ void __launch_bounds__(64 * sk_w(CODE), 8) k_verify_table_shared(Acc 
 // U_(a,j) += S_a[r] T_j[r] over the NP rows of T (one wavefront per item), in
 // the transposed layout: lanes own fold items (j, word pair gp) and keep
 // U_(a,j) for every a in registers (2K per item); a row's scalars S_a[r] are
-// uniform, so their masks are SGPR masks, and the lanes' T words of a row are
-// K MW consecutive words. Rows are read FOLD_AHEAD ahead of their use.
+// uniform across the wave and select each product from the 16 multiples of
+// the lane's T words, and the lanes' T words of a row are K MW consecutive
+// words. Rows are read FOLD_AHEAD ahead of their use.
 // U_(a,j) + U_(j,a) is the pair sum of the bitsliced kernels
 // (sign.cu:395-437, verify.cu:72-182).
 constexpr int FOLD_AHEAD = 8;
@@ -540,28 +543,6 @@ struct SkFold {
     uint32_t t[MW];             // t, read before the fold
     uint32_t U[PAIRS * MW];     // pair sums, nibble-packed
     uint32_t yu[YG_MAX * UW];   // unreduced y
-};
-
-// s x for the 16 scalars s (bit b of s is the coefficient of a^b), held in
-// registers; an index uniform across the wavefront selects one without
-// leaving them
-struct SixteenMultiples {
-    typedef uint32_t v16 __attribute__((ext_vector_type(16)));
-    v16 m;
-    __device__ __forceinline__ explicit SixteenMultiples(uint32_t x) {
-        using gf16::planar::mul_a;
-        const uint32_t x2 = mul_a(x), x4 = mul_a(x2), x8 = mul_a(x4);
-        m[0] = 0u;
-        m[1] = x;
-        m[2] = x2;
-        m[3] = x ^ x2;
-        m[4] = x4;
-        m[5] = x ^ x4;
-        m[6] = x2 ^ x4;
-        m[7] = m[3] ^ x4;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) m[8 + q] = m[q] ^ x8;
-    }
 };
 
 // One row of the fold for one fold item: U[a][h] += S_a[r] x_h for the K
