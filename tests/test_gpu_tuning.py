@@ -190,16 +190,21 @@ def test_autotune_keeps_a_compiled_variant_and_the_results(engine_factory, resto
     assert torch.equal(again, ref)
 
 
+@pytest.mark.parametrize("shared", (False, True), ids=("wave-kernels", "shared-key-kernels"))
 @pytest.mark.parametrize("forced", (0, 2), ids=("plain", "forced-retries"))
-def test_graph_replays_match_eager(engine_factory, forced):
+def test_graph_replays_match_eager(engine_factory, restore_config, forced, shared):
     """Sign and verify pipelines captured as HIP graphs, replayed three
     times with eager work (clearing the outputs) and a synchronisation
     between replays: every replay gives the eager bytes, also when every
     item goes through the retry pass (its failure list is reset inside the
-    kernels, not by a memory-set node)."""
+    kernels, not by a memory-set node), with the wave kernels and with the
+    shared-key kernels (key sort, table and fold kernels)."""
     import torch
     v = 1
-    eng = engine_factory(v)
+    eng = restore_config(engine_factory(v))
+    if shared:
+        code = [p for p in eng.launch_variants()["sign_prep"] if p >= 1000][0]
+        eng.set_launch_config(sign_prep=code, verify=code, sign_chunks=1)
     z = eng.sizes
     n = 1536
     b = _batch(eng, n, 6, 3 + forced, maxlen=900)
@@ -216,7 +221,7 @@ def test_graph_replays_match_eager(engine_factory, forced):
     sigs = torch.zeros_like(ref)
     ws = eng.sign_workspace(n)
     res = torch.full((n,), 7, dtype=torch.uint8, device=eng.device)
-    print(f"[tuning] graphs MAYO_{v} n={n} forced={forced}", flush=True)
+    print(f"[tuning] graphs MAYO_{v} n={n} forced={forced} shared={shared}", flush=True)
     gs = eng.sign_graph(b["pesk"], b["k"], b["m"], b["o"], b["l"], sigs, ws, stages=stages)
     gv = eng.verify_graph(b["pepk"], b["k"], b["m"], b["o"], b["l"], sigs, res, eng.verify_workspace(n))
     try:
