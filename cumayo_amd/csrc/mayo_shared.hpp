@@ -656,8 +656,9 @@ __device__ __forceinline__ void fold_rows(const uint64_t *scol, const uint32_t *
 }
 
 // The pair sums of the fold items' ordered products into Ul (nibble words):
-// U_(a,j) + U_(j,a) for a < j, U_(a,a). Lanes of fold items past the last
-// hold zeros and add nothing.
+// U_(a,j) + U_(j,a) for a < j, U_(a,a). The planar words are summed first
+// (a sum is the same XOR in either layout) and each pair sum is turned into
+// nibble form once. Lanes of fold items past the last add nothing.
 __device__ __forceinline__ void sk_pair_sums(uint32_t *Ul, const uint32_t (&U)[SK_IPL][K][SK_MWL]) {
     const int lane = lane_here();
     for (int e = lane; e < PAIRS * MW; e += 64) Ul[e] = 0;
@@ -671,10 +672,11 @@ __device__ __forceinline__ void sk_pair_sums(uint32_t *Ul, const uint32_t (&U)[S
         for (int a = 0; a < K; ++a) {
             const int p = a <= j ? pair_index(a, j) : pair_index(j, a);
 #pragma unroll
-            for (int h = 0; h < SK_MWL; ++h)
-                atomicXor(&Ul[p * MW + 2 * gp + h], gf16::planar::to_nibbles(U[i][a][h]));
+            for (int h = 0; h < SK_MWL; ++h) atomicXor(&Ul[p * MW + 2 * gp + h], U[i][a][h]);
         }
     }
+    WaveSync::sync();
+    for (int e = lane; e < PAIRS * MW; e += 64) Ul[e] = gf16::planar::to_nibbles(Ul[e]);
     WaveSync::sync();
 }
 
