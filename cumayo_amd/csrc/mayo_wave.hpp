@@ -1104,28 +1104,29 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                 for (int d = 0; d < 4; ++d)
 #pragma unroll
                     for (int h = 0; h < NH; ++h) fw[d][h] &= ~wm[h];
-                // bit masks of c[pr] for every column of set s: the pivot
-                // row's word picked by uniform selects (no branch), then a
-                // sign-extending one-bit extract
-                uint32_t wsel[NG];  // all ones for the pivot row's word
+                // bit masks of c[pr] for every column of the sets this step
+                // reads (the pivot set and the sets it updates): one
+                // sign-extending one-bit extract per plane from the pivot
+                // row's word, chosen by a uniform branch on that word's
+                // index (selects on it would cost two more ops per plane)
+                uint32_t cmall[NCL][4];
+                auto masks_at = [&](auto Gc) {
+                    constexpr int g = decltype(Gc)::value;
+                    static_for<NCL>([&](auto Sc) {
+                        constexpr int s = decltype(Sc)::value;
+                        if constexpr (s != ps && s != Y_SET && set_rank(s) < ord) return;
 #pragma unroll
-                for (int g = 0; g < NG; ++g) wsel[g] = pg == g ? 0xFFFFFFFFu : 0u;
-                auto row_masks = [&](auto Sc, uint32_t (&cm)[4]) {
-                    constexpr int s = decltype(Sc)::value;
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) {
-                        uint32_t w = cols[s][d][0];
-                        // w = wsel[g] ? word g : w, as v_bitop3 (a select
-                        // chain on a uniform index becomes an indexed
-                        // scratch load)
-#pragma unroll
-                        for (int g = 1; g < NG; ++g) w = __builtin_amdgcn_bitop3_b32(w, cols[s][d][g], wsel[g], 0xD8);
-                        cm[d] = (uint32_t)__builtin_amdgcn_sbfe((int)w, pb, 1u);
-                    }
+                        for (int d = 0; d < 4; ++d)
+                            cmall[s][d] = (uint32_t)__builtin_amdgcn_sbfe((int)cols[s][d][g], pb, 1u);
+                    });
                 };
+                static_for<NG>([&](auto Gc) {
+                    if (pg == decltype(Gc)::value) masks_at(Gc);
+                });
                 // the pivot value p = f[pr], read from the pivot lane's masks
                 uint32_t cmp[4];
-                row_masks(std::integral_constant<int, ps>{}, cmp);
+#pragma unroll
+                for (int d = 0; d < 4; ++d) cmp[d] = cmall[ps][d];
                 const uint32_t pv = (cmp[0] & 1u) | (cmp[1] & 2u) | (cmp[2] & 4u) | (cmp[3] & 8u);
                 const uint32_t p = __builtin_amdgcn_readlane(pv, pl);
                 const uint32_t pinv = (uint32_t)(IM_TABLE[0] >> (4 * p)) & 0xFu;
@@ -1178,12 +1179,8 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                     // free columns do not enter x), unless they hold y'
                     if constexpr (s != Y_SET && set_rank(s) < ord) return;
                     uint32_t cm[4];
-                    if constexpr (s == ps) {
 #pragma unroll
-                        for (int d = 0; d < 4; ++d) cm[d] = cmp[d];
-                    } else {
-                        row_masks(Sc, cm);
-                    }
+                    for (int d = 0; d < 4; ++d) cm[d] = cmall[s][d];
 #pragma unroll
                     for (int g = 0; g < NG; ++g)
 #pragma unroll
