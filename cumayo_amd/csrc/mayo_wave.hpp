@@ -994,6 +994,18 @@ __device__ __forceinline__ void scale_planes(uint32_t sc, const uint64_t (&x)[4]
     }
 }
 
+// A uniform value as a VGPR operand (the compiler would otherwise read it
+// from its SGPR).
+__device__ __forceinline__ uint32_t vgpr_copy(uint32_t x) {
+    uint32_t r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+#ifndef MAYO_SOLVE_VG
+#define MAYO_SOLVE_VG 1
+#endif
+constexpr bool SOLVE_VG = MAYO_SOLVE_VG != 0;
+
 // Gauss-Jordan on the columns of [A | y - A r]; x (S.x) = r + z. Returns 1
 // on full rank. Each pivot step: the pivot column f is broadcast (SGPRs),
 // its pivot row pr is the lowest unused row with a nonzero entry, p = f[pr];
@@ -1172,28 +1184,36 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
                         G[b][2][h] = G[b - 1][1][h];
                         G[b][3][h] = G[b - 1][2][h];
                     }
-                static_for<NCL>([&](auto Sc) {
-                    constexpr int s = decltype(Sc)::value;
-                    // sets whose columns all precede pc need no update (pivot
-                    // columns are p e_pr, which the later steps leave alone;
-                    // free columns do not enter x), unless they hold y'
-                    if constexpr (s != Y_SET && set_rank(s) < ord) return;
-                    uint32_t cm[4];
 #pragma unroll
-                    for (int d = 0; d < 4; ++d) cm[d] = cmall[s][d];
+                for (int g = 0; g < NG; ++g) {
+                    // word g of the multiples as VGPR operands (7 distinct of
+                    // the 16): AND-XORs that read no SGPR can issue paired
+                    // with another wave's VALU op
+                    uint32_t vg[4][4];
 #pragma unroll
-                    for (int g = 0; g < NG; ++g)
+                    for (int d = 0; d < 4; ++d)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const uint64_t gw = G[d][e][g >> 1];
+                            vg[d][e] = SOLVE_VG ? vgpr_copy((g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw)
+                                                : ((g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw);
+                        }
+                    static_for<NCL>([&](auto Sc) {
+                        constexpr int s = decltype(Sc)::value;
+                        // sets whose columns all precede pc need no update
+                        // (pivot columns are p e_pr, which the later steps
+                        // leave alone; free columns do not enter x), unless
+                        // they hold y'
+                        if constexpr (s != Y_SET && set_rank(s) < ord) return;
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             uint32_t c = cols[s][e][g];
 #pragma unroll
-                            for (int d = 0; d < 4; ++d) {
-                                const uint64_t gw = G[d][e][g >> 1];
-                                c = gf16::axm(c, (g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw, cm[d]);
-                            }
+                            for (int d = 0; d < 4; ++d) c = gf16::axm(c, vg[d][e], cmall[s][d]);
                             cols[s][e][g] = c;
                         }
-                });
+                    });
+                }
                 if (lane == 0) S.pinv[pc] = (uint8_t)pinv;
 #pragma unroll
                 for (int h = 0; h < NH; ++h) used[h] |= wm[h];
