@@ -509,25 +509,30 @@ def main():
             g_verify.launch()
         torch.cuda.synchronize(dev)
         graph_same = bool(torch.equal(e_sigs, d_sigs)) and bool(torch.equal(e_res, d_res))
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            g_sign.launch()
-        torch.cuda.synchronize(dev)
-        graph_sign_s = (time.perf_counter() - t0) / args.steps
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            g_verify.launch()
-        torch.cuda.synchronize(dev)
-        graph_verify_s = (time.perf_counter() - t0) / args.steps
+        # graph and eager passes alternate, each timed with events on the
+        # stream, so that a drift of the clocks over the run affects both
+        # alike; medians over the rounds
+        rounds = max(args.steps, 5)
+        ga, gb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def ev_ms(fn):
+            ga.record(stream)
+            fn()
+            gb.record(stream)
+            gb.synchronize()
+            return ga.elapsed_time(gb)
+
+        t_gs, t_es, t_gv, t_ev = [], [], [], []
+        for _ in range(rounds):
+            t_gs.append(ev_ms(lambda: g_sign.launch()))
+            t_es.append(ev_ms(sign_step))
+            t_gv.append(ev_ms(lambda: g_verify.launch()))
+            t_ev.append(ev_ms(verify_step))
+        graph_sign_s = float(np.median(t_gs)) / 1e3
+        graph_verify_s = float(np.median(t_gv)) / 1e3
+        eager_after_s = float(np.median(t_es)) / 1e3
+        eager_verify_after_s = float(np.median(t_ev)) / 1e3
         graph_same = graph_same and bool(torch.equal(e_sigs, d_sigs)) and bool(torch.equal(e_res, d_res))
-        # the eager sign pass again, right after the graph replays: the
-        # graph's rate is compared with the eager rate of the same moment
-        torch.cuda.synchronize(dev)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            sign_step()
-        torch.cuda.synchronize(dev)
-        eager_after_s = (time.perf_counter() - t0) / args.steps
         g_sign.close()
         g_verify.close()
         del e_sigs, e_res
@@ -573,8 +578,10 @@ def main():
                                         "verify_per_s": round(n / ref_verify_s, 1),
                                         "identical_outputs": ref_same},
             "graph": {"pipelines": "sign and verify passes captured as HIP graphs, one graph launch each",
+                      "timing": "graph and eager passes alternating, events on the stream, medians",
                       "sign_per_s": round(n / graph_sign_s, 1), "verify_per_s": round(n / graph_verify_s, 1),
                       "eager_sign_per_s_after": round(n / eager_after_s, 1),
+                      "eager_verify_per_s_after": round(n / eager_verify_after_s, 1),
                       "identical_outputs": graph_same},
         }
     # restore the workspace of the default form for the attempt counts below
