@@ -1399,7 +1399,9 @@ __device__ __noinline__ void retry_linear_algebra(Shared &S, int ctr, bool force
 // wave 0 re-derives V || r and runs the linear algebra. Grid: any size; a
 // workgroup takes list entries blockIdx.x, + gridDim.x, ... of the failure
 // list kept at item `list` (the first item of the launch part that made it).
-constexpr int RNW_MAX = M <= 80 ? 8 : 4;  // larger sets: a whole SIMD's registers per wave
+// Eight waves, two per SIMD: the kernel needs at most 243 VGPRs (MAYO_5),
+// within the 256 that two waves per SIMD leave each.
+constexpr int RNW_MAX = 8;
 constexpr int RNW = qf_chunks<O, false>() < RNW_MAX ? qf_chunks<O, false>() : RNW_MAX;
 template <class Acc, class KL>
 __global__ void __launch_bounds__(64 * RNW, 1) k_sign_retry(Acc acc, size_t n, int fail_first = 0,
