@@ -1001,11 +1001,6 @@ __device__ __forceinline__ uint32_t vgpr_copy(uint32_t x) {
     asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
     return r;
 }
-#ifndef MAYO_SOLVE_VG
-#define MAYO_SOLVE_VG 1
-#endif
-constexpr bool SOLVE_VG = MAYO_SOLVE_VG != 0;
-
 // Gauss-Jordan on the columns of [A | y - A r]; x (S.x) = r + z. Returns 1
 // on full rank. Each pivot step: the pivot column f is broadcast (SGPRs),
 // its pivot row pr is the lowest unused row with a nonzero entry, p = f[pr];
@@ -1195,8 +1190,7 @@ __device__ int solve(LDS &S, uint32_t (&cols)[NCL][4][NG], bool force_fail = fal
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             const uint64_t gw = G[d][e][g >> 1];
-                            vg[d][e] = SOLVE_VG ? vgpr_copy((g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw)
-                                                : ((g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw);
+                            vg[d][e] = vgpr_copy((g & 1) ? (uint32_t)(gw >> 32) : (uint32_t)gw);
                         }
                     static_for<NCL>([&](auto Sc) {
                         constexpr int s = decltype(Sc)::value;
