@@ -158,7 +158,7 @@ def test_table_form_large_batch_includes_retries(engine_factory):
     assert np.array_equal(psigs.cpu().numpy(), g)
 
 
-@pytest.mark.parametrize("v", (1, 5), ids=lambda v: f"MAYO_{v}")
+@pytest.mark.parametrize("v", (1, 3, 5), ids=lambda v: f"MAYO_{v}")
 def test_retry_pass_many_items(engine_factory, v):
     """Forced rank failures of attempts 0 and 1 for 300 items: every item
     goes through the retry pass (more items than its workgroups, so they
